@@ -1,0 +1,215 @@
+/*
+ * laker_hip.h -- C ABI of the MI355X-native LAKER solver core.
+ *
+ * Plain pointers, sizes and hipStream_t only.  All device pointers are
+ * caller-owned except the opaque handles, whose internal device buffers
+ * (packed embeddings, preconditioner factors, Krylov workspaces) are
+ * allocated by the library at create time and released by *_destroy.
+ * Every call is stream-ordered on the given stream; calls on one handle must
+ * not overlap.  Return value: LAKER_OK or a negative LAKER_E* code, with the
+ * message available from laker_last_error() (thread-local).
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference repository root).
+ */
+#ifndef LAKER_HIP_H
+#define LAKER_HIP_H
+
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (Python maps EINVAL->ValueError, the rest->RuntimeError) */
+#define LAKER_OK 0
+#define LAKER_EINVAL -1     /* bad shape / argument */
+#define LAKER_EHIP -2       /* HIP runtime error */
+#define LAKER_ENOMEM -3     /* device allocation failed */
+#define LAKER_EBREAKDOWN -4 /* PCG non-positive curvature (laker/solvers.py:232-246) */
+#define LAKER_ESTATE -5     /* object not built (laker/preconditioner.py:303-304) */
+#define LAKER_ENUMERIC -6   /* factorisation failure (non-SPD small matrix) */
+
+/* ---- element types */
+#define LAKER_F32 0
+#define LAKER_F64 1
+
+/* ---- matvec arithmetic */
+#define LAKER_PREC_FP32 0  /* Gram on FP32 MFMA, exp + accumulation in FP32 */
+#define LAKER_PREC_MIXED 1 /* Gram on BF16 MFMA from a hi/lo split of E (4 products), FP32 accumulate */
+#define LAKER_PREC_FP64 2  /* Gram, exp and accumulation in FP64 (refinement residual) */
+#define LAKER_PREC_BF16 3  /* BF16 Gram and BF16 G.X products (probe matvec of the CCCP build) */
+
+/* ---- clamp policy for exp(min(s*g, 80|700)) (laker/kernels.py:80-113,181-185) */
+#define LAKER_CLAMP_AUTO 0   /* clamp unless max_i ||e_i||^2 * scale < 80 (fp32) / 700 (fp64) */
+#define LAKER_CLAMP_ALWAYS 1 /* always clamp (tiled predict, laker/core.py:653) */
+#define LAKER_CLAMP_NEVER 2  /* never clamp (to_dense, laker/kernels.py:296-298) */
+
+typedef struct laker_operator_s* laker_operator_t;
+typedef struct laker_precond_s* laker_precond_t;
+
+/* Text of the last error on this thread ("" if none). */
+const char* laker_last_error(void);
+/* Library version string. */
+const char* laker_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Exact attention-kernel operator  A = lambda I + exp(scale * E E^T)
+ * Replaces AttentionKernelOperator.__init__ (laker/kernels.py:154-185).
+ * E: n x d row-major (ld = d), dtype LAKER_F32 or LAKER_F64, device memory.
+ * The handle copies and packs E (FP32 MFMA operands, BF16 hi/lo split
+ * operands, FP64 copy) and evaluates the skip-clamp predicate on device.
+ */
+int laker_operator_create(const void* E, int64_t n, int32_t d, int32_t dtype, double lambda_reg,
+                          double scale, int32_t clamp_policy, hipStream_t stream,
+                          laker_operator_t* out);
+int laker_operator_destroy(laker_operator_t op);
+/* skip_clamp flag (laker/kernels.py:183-185) and max squared row norm. */
+int laker_operator_info(laker_operator_t op, int32_t* skip_clamp, double* max_sq_norm);
+/* Change lambda without repacking (regularisation path / bilevel use). */
+int laker_operator_set_lambda(laker_operator_t op, double lambda_reg);
+
+/* Y = (lambda I + G) X.  X, Y: n x k, column stride... row-major with leading
+ * dimension ldx/ldy (elements), dtype = operator dtype.
+ * Replaces AttentionKernelOperator.matvec / matvec_impl (laker/kernels.py:187-270)
+ * and dense_attention_matvec (laker/kernels.py:121-130). */
+int laker_matvec(laker_operator_t op, const void* X, int32_t k, int64_t ldx, void* Y, int64_t ldy,
+                 int32_t precision, hipStream_t stream);
+
+/* R = B - (lambda I + G) X with FP64 X/B/R and FP64 Gram/exp/accumulation:
+ * the iterative-refinement residual (no reference counterpart; it checks the
+ * reference's stopping quantity ||b - A x|| / ||b||, laker/solvers.py:266-267). */
+int laker_residual_fp64(laker_operator_t op, const double* B, const double* X, int32_t k,
+                        double* R, hipStream_t stream);
+
+/* Y = exp(min(scale * Eq E^T, clamp)) X for m query rows (no lambda term).
+ * Replaces the predict path kernel_eval(q, E) @ alpha (laker/core.py:580-661,
+ * laker/kernels.py:301-354) without materialising m x n.
+ * clamp_policy LAKER_CLAMP_AUTO uses the operator's skip-clamp flag. */
+int laker_cross_matvec(laker_operator_t op, const void* Eq, int64_t m, const void* X, int32_t k,
+                       int64_t ldx, void* Y, int64_t ldy, int32_t clamp_policy, int32_t precision,
+                       hipStream_t stream);
+
+/* Transposed cross product Y = exp(min(scale * E Eq^T, clamp)) Xq  (n x k),
+ * Xq: m x k.  Used by the hand-written backward of predict and the batched
+ * variance right-hand sides. */
+int laker_cross_matvec_t(laker_operator_t op, const void* Eq, int64_t m, const void* Xq, int32_t k,
+                         int64_t ldx, void* Y, int64_t ldy, int32_t clamp_policy,
+                         hipStream_t stream);
+
+/* K = exp(min(scale * X Y^T, clamp)) materialised (m x p, row-major, ldk).
+ * Replaces AttentionKernelOperator.kernel_eval (laker/kernels.py:301-354). */
+int laker_kernel_eval(const void* X, int64_t m, const void* Y, int64_t p, int32_t d, int32_t dtype,
+                      double scale, int32_t clamp, void* K, int64_t ldk, hipStream_t stream);
+
+/* out[i] = lambda + exp(scale * ||e_i||^2)  (no clamp).
+ * Replaces AttentionKernelOperator.diagonal (laker/kernels.py:272-283). */
+int laker_diagonal(laker_operator_t op, void* out, hipStream_t stream);
+
+/* Dense lambda I + exp(scale E E^T) (no clamp), n x n row-major.
+ * Replaces AttentionKernelOperator.to_dense (laker/kernels.py:285-299). */
+int laker_to_dense(laker_operator_t op, void* out, hipStream_t stream);
+
+/* Hand-written backward of Y = lambda X + G X with G = exp(min(s E E^T, c)):
+ * given dY (n x k), writes dX = lambda dY + G dY and
+ * dE_i = s * sum_j G'_ij (dY_i . X_j + X_i . dY_j) E_j  (G' = 0 where clamped).
+ * Tiles are recomputed, nothing n x n is stored.  Replaces autograd through
+ * exp_safe's out-of-place branch (laker/kernels.py:100-101,111-112) used by
+ * laker/training.py:186-211.  dX or dE may be NULL to skip. */
+int laker_matvec_backward(laker_operator_t op, const float* X, const float* dY, int32_t k,
+                          float* dX, float* dE, hipStream_t stream);
+
+/* Backward of Yq = exp(min(s Eq E^T, c)) X  (predict_train, laker/core.py:782-849):
+ * dEq (m x d), dE (n x d) accumulated (+=), dX (n x k) written.  Any may be NULL. */
+int laker_cross_matvec_backward(laker_operator_t op, const float* Eq, int64_t m, const float* X,
+                                int32_t k, const float* dYq, float* dEq, float* dE, float* dX,
+                                int32_t clamp_policy, hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * CCCP preconditioner  P = Sigma^{-1/2},  Sigma = a I + Q C Q^T
+ * Replaces CCCPPreconditioner.build (laker/preconditioner.py:144-284).
+ * probes: n x nr row-major in the operator dtype (device), or NULL to draw
+ * N(0,1) probes on device from `seed`.  probe_precision selects the matvec
+ * used for the probe responses (LAKER_PREC_FP32 or LAKER_PREC_BF16 ...).
+ * The QR is a shifted Cholesky-QR3 in FP64; the N_r x N_r CCCP loop, its
+ * Cholesky factorisations and the final inverse square root run in FP64 on
+ * device.  *iterations receives the CCCP iteration count. */
+int laker_precond_build(laker_operator_t op, int32_t nr, double gamma, double epsilon,
+                        double base_rho, int32_t max_iter, double tol, const void* probes,
+                        uint64_t seed, int32_t probe_precision, hipStream_t stream,
+                        laker_precond_t* out, int32_t* iterations);
+/* Same build from probe responses U = A R computed by the caller (n x nr,
+ * dtype LAKER_F32/LAKER_F64, device): the path for operators that are Python
+ * callables (CCCPPreconditioner.build(operator, n) with a generic operator,
+ * laker/preconditioner.py:186-284). */
+int laker_precond_build_from_responses(const void* U, int64_t n, int32_t nr, int32_t dtype,
+                                       double gamma, double epsilon, double base_rho, int32_t max_iter,
+                                       double tol, hipStream_t stream, laker_precond_t* out,
+                                       int32_t* iterations);
+int laker_precond_destroy(laker_precond_t pc);
+/* Copy a factor into caller memory: which = 0 Q (n x nr, op dtype), 1 R, 2 C,
+ * 3 S (nr x nr FP64).  Backs the q_basis / qr_r / q_basis_matrix attributes
+ * (laker/preconditioner.py:137-142). */
+int laker_precond_copy_out(laker_precond_t pc, int32_t which, void* dst, hipStream_t stream);
+/* isotropic_coef a, N_r, shrinkage rho; device pointers to Q (n x nr, op dtype),
+ * R of the QR (nr x nr FP64), C (nr x nr FP64) and S = M^{-1/2} - a^{-1/2} I (FP64). */
+int laker_precond_info(laker_precond_t pc, double* isotropic_coef, int32_t* nr, double* rho,
+                       const void** q_basis, const double** qr_r, const double** c_matrix,
+                       const double** s_matrix);
+/* Z = P X.  Replaces CCCPPreconditioner.apply / preconditioner_apply_core
+ * (laker/preconditioner.py:52-68,286-352). */
+int laker_precond_apply(laker_precond_t pc, const void* X, int32_t k, int64_t ldx, void* Z,
+                        int64_t ldz, hipStream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Device-resident PCG.  Replaces PreconditionedConjugateGradient.solve /
+ * solve_1d / solve_2d (laker/solvers.py:130-394) when both the operator and
+ * the preconditioner are native.  B, X: n x k (op dtype).  X0 may be NULL.
+ * k == 1 follows solve_1d, k > 1 follows solve_2d (per-column alpha/beta,
+ * global Frobenius stopping ratio, breakdown if any column).
+ * check_every: host polls the device status every that many iterations. */
+typedef struct laker_pcg_result {
+  int32_t iterations;    /* reference `iterations` */
+  int32_t status;        /* LAKER_OK, LAKER_EBREAKDOWN */
+  int32_t converged;     /* 1 if ||r||/||b|| <= tol was reached */
+  int32_t zero_rhs;      /* ||b|| == 0 early return (laker/solvers.py:181-182) */
+  double residual_norm;  /* reference `residual_norm` (recurrence residual) */
+  double rhs_norm;
+} laker_pcg_result;
+
+int laker_pcg_solve(laker_operator_t op, laker_precond_t pc, const void* B, int32_t k, void* X,
+                    const void* X0, double tol, int32_t max_iter, double breakdown_eps,
+                    int32_t restart_freq, int32_t precision, int32_t check_every,
+                    hipStream_t stream, laker_pcg_result* result);
+
+/* Mixed-precision solve: inner PCG with BF16-MFMA Gram tiles (LAKER_PREC_MIXED)
+ * and FP32 vectors, outer FP64 iterative refinement on the FP64 residual.
+ * B, X: n x k FP64.  Stops when ||B - A X||_F / ||B||_F <= tol in FP64.
+ * No reference counterpart (new precision mode behind the same solve call). */
+typedef struct laker_mixed_result {
+  int32_t outer_iterations;
+  int32_t inner_iterations; /* total inner PCG iterations */
+  int32_t status;
+  int32_t converged;
+  double residual_norm;     /* FP64 true residual ||B - A X||_F */
+  double rhs_norm;
+} laker_mixed_result;
+
+int laker_solve_mixed(laker_operator_t op, laker_precond_t pc, const double* B, int32_t k,
+                      double* X, double tol, int32_t max_outer, double inner_tol,
+                      int32_t inner_max_iter, int32_t inner_precision, hipStream_t stream,
+                      laker_mixed_result* result);
+
+/* ---------------------------------------------------------------------------
+ * Column-wise vector kernels for the generic PCG path, where operator or
+ * preconditioner are Python callables (laker/solvers.py:225-286, 334-385).
+ * out[c] (FP64, device) = sum_i X[i,c] * Y[i,c] for c < k; X, Y n x k
+ * row-major with leading dimension ld, dtype LAKER_F32/LAKER_F64. */
+int laker_coldot(const void* X, const void* Y, int64_t n, int32_t k, int64_t ld, int32_t dtype,
+                 double* out, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LAKER_HIP_H */

@@ -1,0 +1,684 @@
+// CCCP preconditioner P = Sigma^{-1/2}, Sigma = a I + Q C Q^T, built and applied on device.
+//
+// Build (restates laker/preconditioner.py:144-284):
+//   U = A R (probe matvec, k = N_r), column normalisation clamped at epsilon,
+//   QR by Cholesky-QR2 in FP64 (shifted Cholesky-QR3 when the first Gram
+//   factorisation fails), then the shrinkage-regularised CCCP fixed point in
+//   FP64 with diag(R^T M^{-1} R) taken from a Cholesky factor of M = aI + C
+//   (M = t F_rho is SPD, so this equals the reference's clamped eigh while the
+//   clamp is inactive), and finally S = M^{-1/2} - a^{-1/2} I by a coupled
+//   Newton-Schulz iteration (GEMMs only).
+// Apply (restates laker/preconditioner.py:52-68):
+//   Z = a^{-1/2} X + Q (S (Q^T X)), two HBM-bound passes over Q.
+#include <cmath>
+#include <memory>
+#include <vector>
+
+#include "dense64.h"
+#include "laker_common.h"
+#include "laker_internal.h"
+
+namespace laker {
+
+struct Precond {
+  int64_t n = 0;
+  int nr = 0;
+  int dtype = LAKER_F32;
+  double a = 1.0, rho = 0.0, inv_sqrt_a = 1.0;
+  DevBuf Q, R, C, S;
+  DevBuf w1, w2, part, ws, scal;
+};
+
+// ---------------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------------
+
+// splitmix64-based standard normals (Box-Muller); used only when the caller
+// passes no probe matrix.
+__device__ inline uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+template <class T>
+__global__ void randn_kernel(T* __restrict__ out, int64_t count, uint64_t seed) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint64_t a = splitmix64(seed * 0x632BE59BD9B4E019ull + 2 * (uint64_t)i);
+  const uint64_t b = splitmix64(seed * 0x632BE59BD9B4E019ull + 2 * (uint64_t)i + 1);
+  const double u1 = ((a >> 11) + 1.0) * (1.0 / 9007199254740993.0);
+  const double u2 = (b >> 11) * (1.0 / 9007199254740992.0);
+  out[i] = (T)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+}
+
+// column sums of squares of an n x m row-major matrix -> out[m] (FP64), grid = m
+template <class T>
+__global__ void colsumsq_kernel(const T* __restrict__ U, int64_t n, int m, double* __restrict__ out) {
+  __shared__ double sm[16];
+  const int c = blockIdx.x;
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const double v = (double)U[i * m + c];
+    s += v * v;
+  }
+  s = block_sum(s, sm);
+  if (threadIdx.x == 0) out[c] = s;
+}
+
+// U[:, c] /= max(sqrt(ss[c]), eps)  (in U's dtype, like the reference)
+template <class T>
+__global__ void colscale_kernel(T* __restrict__ U, int64_t n, int m, const double* __restrict__ ss,
+                                double eps) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * m) return;
+  const int c = (int)(idx % m);
+  const T nrm = (T)sqrt(ss[c]);
+  const T den = nrm > (T)eps ? nrm : (T)eps;
+  U[idx] = U[idx] / den;
+}
+
+template <class TI, class TO>
+__global__ void convert_kernel(const TI* __restrict__ in, TO* __restrict__ out, int64_t count) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) out[i] = (TO)in[i];
+}
+
+__global__ void add_diag_kernel(double* __restrict__ A, int64_t n, double v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) A[i * n + i] += v;
+}
+
+// M = C + a I
+__global__ void form_m_kernel(const double* __restrict__ C, double* __restrict__ M, int64_t n,
+                              double a) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  M[idx] = C[idx] + ((idx / n == idx % n) ? a : 0.0);
+}
+
+// w[c] = (n/nr) / (sum_r Z[r][c]^2 + eps);  Rw[r][c] = R[r][c] * w[c]; grid = nr
+__global__ void cccp_weights_kernel(const double* __restrict__ Z, const double* __restrict__ R,
+                                    double* __restrict__ Rw, int nr, double n_over_nr, double eps) {
+  __shared__ double sm[16];
+  const int c = blockIdx.x;
+  double s = 0.0;
+  for (int r = threadIdx.x; r < nr; r += blockDim.x) {
+    const double v = Z[(int64_t)r * nr + c];
+    s += v * v;
+  }
+  s = block_sum(s, sm);
+  const double w = n_over_nr / (s + eps);
+  for (int r = threadIdx.x; r < nr; r += blockDim.x) Rw[(int64_t)r * nr + c] = R[(int64_t)r * nr + c] * w;
+}
+
+// Scalars of one CCCP iteration, kept on device.
+struct CccpScalars {
+  double trace;   // tr(F_rho)
+  double t, a_new;
+  double diff2, c2; // ||C_new - C||_F^2, ||C||_F^2
+};
+
+// trace of F_rho = sum_i ((1-rho)(H_ii + gamma)/s + rho); single block
+__global__ void cccp_trace_kernel(const double* __restrict__ H, int nr, double gamma, double s,
+                                  double rho, double a_rho, int64_t n, CccpScalars* sc) {
+  __shared__ double sm[16];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < nr; i += blockDim.x)
+    v += (1.0 - rho) * (H[(int64_t)i * nr + i] + gamma) / s + rho;
+  v = block_sum(v, sm);
+  if (threadIdx.x == 0) {
+    sc->trace = v;
+    sc->t = (double)n / (a_rho * (double)(n - nr) + v);
+    sc->a_new = sc->t * a_rho;
+  }
+}
+
+// C_new = t ((1-rho)(H + gamma I)/s + rho I) - a_new I; partial norms per block
+__global__ void cccp_update_kernel(const double* __restrict__ H, const double* __restrict__ C,
+                                   double* __restrict__ Cn, int nr, double gamma, double s,
+                                   double rho, const CccpScalars* __restrict__ sc,
+                                   double* __restrict__ part) {
+  __shared__ double sm[16];
+  const double t = sc->t, an = sc->a_new;
+  const int64_t total = (int64_t)nr * nr;
+  double d2 = 0.0, c2 = 0.0;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const bool dg = (idx / nr) == (idx % nr);
+    const double f = (1.0 - rho) * (H[idx] + (dg ? gamma : 0.0)) / s + (dg ? rho : 0.0);
+    const double cn = t * f - (dg ? an : 0.0);
+    const double co = C[idx];
+    Cn[idx] = cn;
+    d2 += (cn - co) * (cn - co);
+    c2 += co * co;
+  }
+  d2 = block_sum(d2, sm);
+  c2 = block_sum(c2, sm);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = d2;
+    part[2 * blockIdx.x + 1] = c2;
+  }
+}
+
+__global__ void cccp_norms_final_kernel(const double* __restrict__ part, int nb, CccpScalars* sc) {
+  if (threadIdx.x == 0) {
+    double d2 = 0.0, c2 = 0.0;
+    for (int b = 0; b < nb; ++b) {
+      d2 += part[2 * b];
+      c2 += part[2 * b + 1];
+    }
+    sc->diff2 = d2;
+    sc->c2 = c2;
+  }
+}
+
+// ||A||_F^2 (single block) -> out
+__global__ void frob2_kernel(const double* __restrict__ A, int64_t count, double* __restrict__ out) {
+  __shared__ double sm[16];
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < count; i += blockDim.x) s += A[i] * A[i];
+  s = block_sum(s, sm);
+  if (threadIdx.x == 0) *out = s;
+}
+
+// Newton-Schulz step helper: given P = Z Y, T = (3I - P)/2 ; err2 = ||P - I||_F^2 (block partials)
+__global__ void ns_t_kernel(const double* __restrict__ P, double* __restrict__ T, int64_t n,
+                            double* __restrict__ part) {
+  __shared__ double sm[16];
+  double e2 = 0.0;
+  const int64_t total = n * n;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const bool dg = (idx / n) == (idx % n);
+    const double p = P[idx];
+    const double dv = p - (dg ? 1.0 : 0.0);
+    e2 += dv * dv;
+    T[idx] = ((dg ? 3.0 : 0.0) - p) * 0.5;
+  }
+  e2 = block_sum(e2, sm);
+  if (threadIdx.x == 0) part[blockIdx.x] = e2;
+}
+
+__global__ void sum_parts_kernel(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += part[b];
+    *out = s;
+  }
+}
+
+// S = Z * f - g I
+__global__ void scale_sub_diag_kernel(const double* __restrict__ Zm, double* __restrict__ S, int64_t n,
+                                      double f, double g) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  S[idx] = Zm[idx] * f - ((idx / n == idx % n) ? g : 0.0);
+}
+
+// S[i][j] = f (Z[i][j] + Z[j][i]) / 2 - g delta_ij
+__global__ void sym_scale_kernel(const double* __restrict__ Zm, double* __restrict__ S, int64_t n,
+                                 double f, double g) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  const int64_t i = idx / n, j = idx % n;
+  S[idx] = 0.5 * f * (Zm[idx] + Zm[j * n + i]) - (i == j ? g : 0.0);
+}
+
+// ---------------------------------------------------------------------------
+// apply kernels (k == 1)
+// ---------------------------------------------------------------------------
+// partial[b][c] = sum_{i in chunk b} Q[i][c] x[i]
+template <class T>
+__global__ __launch_bounds__(256) void qtx_vec_kernel(const T* __restrict__ Q, const T* __restrict__ x,
+                                                      int64_t n, int nr, int64_t rows_per_block,
+                                                      double* __restrict__ part) {
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+  const int64_t r1 = (r0 + rows_per_block < n) ? r0 + rows_per_block : n;
+  for (int c0 = 0; c0 < nr; c0 += 256 * 4) {
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t i = r0; i < r1; ++i) {
+      const double xi = (double)x[i];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int c = c0 + u * 256 + threadIdx.x;
+        if (c < nr) acc[u] = fma((double)Q[i * nr + c], xi, acc[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int c = c0 + u * 256 + threadIdx.x;
+      if (c < nr) part[(int64_t)blockIdx.x * nr + c] = acc[u];
+    }
+  }
+}
+
+__global__ void reduce_parts_cols_kernel(const double* __restrict__ part, int nb, int nr,
+                                         double* __restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nr) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += part[(int64_t)b * nr + c];
+  out[c] = s;
+}
+
+// z[i] = f * x[i] + sum_c Q[i][c] w[c]   (one wave per row)
+template <class T>
+__global__ __launch_bounds__(256) void qw_vec_kernel(const T* __restrict__ Q, const double* __restrict__ w,
+                                                     const T* __restrict__ x, T* __restrict__ z,
+                                                     int64_t n, int nr, double f) {
+  extern __shared__ double ws_lds[];
+  for (int c = threadIdx.x; c < nr; c += blockDim.x) ws_lds[c] = w[c];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= n) return;
+  double s = 0.0;
+  for (int c = lane; c < nr; c += 64) s = fma((double)Q[i * nr + c], ws_lds[c], s);
+  s = wave_sum(s);
+  if (lane == 0) z[i] = (T)(f * (double)x[i] + s);
+}
+
+template <class T>
+__global__ void scale_copy_kernel(const T* __restrict__ X, int64_t ldx, T* __restrict__ Z, int64_t ldz,
+                                  int64_t n, int k, double f) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * k) return;
+  const int64_t i = idx / k;
+  const int c = (int)(idx % k);
+  Z[i * ldz + c] = (T)(f * (double)X[i * ldx + c]);
+}
+
+// ---------------------------------------------------------------------------
+template <class T>
+static int apply_t(Precond& pc, const T* X, int k, int64_t ldx, T* Z, int64_t ldz, hipStream_t st) {
+  const int64_t n = pc.n;
+  const int nr = pc.nr;
+  LAKER_TRY(pc.w1.ensure((size_t)nr * k * 8));
+  LAKER_TRY(pc.w2.ensure((size_t)nr * k * 8));
+  double* W1 = pc.w1.as<double>();
+  double* W2 = pc.w2.as<double>();
+  const T* Q = pc.Q.as<T>();
+  if (k == 1 && ldx == 1 && ldz == 1) {
+    const int64_t nb = std::min<int64_t>(1024, ceil_div(n, 64));
+    const int64_t rpb = ceil_div(n, nb);
+    const int64_t nb_eff = ceil_div(n, rpb);
+    LAKER_TRY(pc.part.ensure((size_t)nb_eff * nr * 8));
+    hipLaunchKernelGGL((qtx_vec_kernel<T>), dim3((unsigned)nb_eff), dim3(256), 0, st, Q, X, n, nr, rpb,
+                       pc.part.as<double>());
+    hipLaunchKernelGGL(reduce_parts_cols_kernel, dim3((unsigned)ceil_div(nr, 256)), dim3(256), 0, st,
+                       pc.part.as<double>(), (int)nb_eff, nr, W1);
+    LAKER_HIP_CHECK(hipGetLastError());
+    LAKER_TRY((gemm<double, double, double>(false, false, nr, 1, nr, 1.0, pc.S.as<double>(), nr, W1,
+                                             1, 0.0, W2, 1, nullptr, st)));
+    hipLaunchKernelGGL((qw_vec_kernel<T>), dim3((unsigned)ceil_div(n, 4)), dim3(256), nr * sizeof(double),
+                       st, Q, W2, X, Z, n, nr, pc.inv_sqrt_a);
+    LAKER_HIP_CHECK(hipGetLastError());
+    return LAKER_OK;
+  }
+  // general k: W1 = Q^T X ; W2 = S W1 ; Z = a^{-1/2} X + Q W2
+  LAKER_TRY((gemm<T, T, double>(true, false, nr, k, n, 1.0, Q, nr, X, ldx, 0.0, W1, k, &pc.ws, st)));
+  LAKER_TRY((gemm<double, double, double>(false, false, nr, k, nr, 1.0, pc.S.as<double>(), nr, W1, k,
+                                           0.0, W2, k, nullptr, st)));
+  hipLaunchKernelGGL((scale_copy_kernel<T>), dim3((unsigned)ceil_div(n * k, 256)), dim3(256), 0, st, X,
+                     ldx, Z, ldz, n, k, pc.inv_sqrt_a);
+  LAKER_HIP_CHECK(hipGetLastError());
+  LAKER_TRY((gemm<T, double, T>(false, false, n, k, nr, 1.0, Q, nr, W2, k, 1.0, Z, ldz, nullptr, st)));
+  return LAKER_OK;
+}
+
+int precond_apply(Precond& pc, const void* X, int k, int64_t ldx, void* Z, int64_t ldz, hipStream_t st) {
+  if (pc.dtype == LAKER_F32) return apply_t<float>(pc, (const float*)X, k, ldx, (float*)Z, ldz, st);
+  return apply_t<double>(pc, (const double*)X, k, ldx, (double*)Z, ldz, st);
+}
+
+static int read_scalar(const void* dev, void* host, size_t bytes, hipStream_t st) {
+  LAKER_HIP_CHECK(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, st));
+  LAKER_HIP_CHECK(hipStreamSynchronize(st));
+  return LAKER_OK;
+}
+
+// One Cholesky-QR pass in FP64: G = V^T V (+ shift I), L = chol(G), V <- V L^{-T}, Racc <- L^T Racc.
+// Returns LAKER_ENUMERIC if the factorisation failed.
+static int cholqr_pass(double* V, int64_t n, int nr, double shift, double* G, double* Linv,
+                       double* Racc, double* tmp, double* Vtmp, int* info, DevBuf* ws,
+                       hipStream_t st) {
+  LAKER_TRY((gemm<double, double, double>(true, false, nr, nr, n, 1.0, V, nr, V, nr, 0.0, G, nr, ws, st)));
+  if (shift > 0.0) {
+    hipLaunchKernelGGL(add_diag_kernel, dim3((unsigned)ceil_div(nr, 256)), dim3(256), 0, st, G, (int64_t)nr,
+                       shift);
+  }
+  LAKER_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int), st));
+  LAKER_TRY(potrf_lower(G, nr, nr, info, ws, st));
+  int hinfo = 0;
+  LAKER_TRY(read_scalar(info, &hinfo, sizeof(int), st));
+  if (hinfo != 0) return LAKER_ENUMERIC;
+  // Linv = L^{-1}
+  LAKER_TRY(set_identity(Linv, nr, 1.0, st));
+  LAKER_TRY(trsm_lower_left(G, nr, nr, Linv, nr, nr, ws, st));
+  // Vtmp = V Linv^T ; copy back
+  LAKER_TRY((gemm<double, double, double>(false, true, n, nr, nr, 1.0, V, nr, Linv, nr, 0.0, Vtmp, nr,
+                                           nullptr, st)));
+  LAKER_HIP_CHECK(hipMemcpyAsync(V, Vtmp, (size_t)n * nr * 8, hipMemcpyDeviceToDevice, st));
+  // Racc <- L^T Racc   (L^T upper = R factor of this pass)
+  LAKER_TRY((gemm<double, double, double>(true, false, nr, nr, nr, 1.0, G, nr, Racc, nr, 0.0, tmp, nr,
+                                           nullptr, st)));
+  LAKER_HIP_CHECK(hipMemcpyAsync(Racc, tmp, (size_t)nr * nr * 8, hipMemcpyDeviceToDevice, st));
+  return LAKER_OK;
+}
+
+// Steps 2-5 of the build on probe responses U (n x nr, op dtype; consumed).
+static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, int nr, double gamma,
+                        double epsilon, double base_rho, int max_iter, double tol, hipStream_t st,
+                        int* iterations) {
+  const int64_t n = pc->n;
+  const size_t esz = pc->dtype == LAKER_F32 ? 4 : 8;
+  const int64_t nn = (int64_t)nr * nr;
+  (void)op;
+  // 2. column normalisation in the solver dtype
+  DevBuf ss;
+  LAKER_TRY(ss.ensure((size_t)nr * 8));
+  if (pc->dtype == LAKER_F32) {
+    hipLaunchKernelGGL((colsumsq_kernel<float>), dim3(nr), dim3(256), 0, st, U.as<float>(), n, nr, ss.as<double>());
+    hipLaunchKernelGGL((colscale_kernel<float>), dim3((unsigned)ceil_div(n * nr, 256)), dim3(256), 0, st,
+                       U.as<float>(), n, nr, ss.as<double>(), epsilon);
+  } else {
+    hipLaunchKernelGGL((colsumsq_kernel<double>), dim3(nr), dim3(256), 0, st, U.as<double>(), n, nr, ss.as<double>());
+    hipLaunchKernelGGL((colscale_kernel<double>), dim3((unsigned)ceil_div(n * nr, 256)), dim3(256), 0, st,
+                       U.as<double>(), n, nr, ss.as<double>(), epsilon);
+  }
+  LAKER_HIP_CHECK(hipGetLastError());
+
+  // 3. Cholesky-QR2 in FP64 (shifted Cholesky-QR3 on failure)
+  DevBuf V, Vtmp, G, Linv, Racc, tmp, infob;
+  LAKER_TRY(V.ensure((size_t)n * nr * 8));
+  LAKER_TRY(Vtmp.ensure((size_t)n * nr * 8));
+  LAKER_TRY(G.ensure(nn * 8));
+  LAKER_TRY(Linv.ensure(nn * 8));
+  LAKER_TRY(Racc.ensure(nn * 8));
+  LAKER_TRY(tmp.ensure(nn * 8));
+  LAKER_TRY(infob.ensure(sizeof(int)));
+  auto load_v = [&]() -> int {
+    if (pc->dtype == LAKER_F32)
+      hipLaunchKernelGGL((convert_kernel<float, double>), dim3((unsigned)ceil_div(n * nr, 256)), dim3(256), 0,
+                         st, U.as<float>(), V.as<double>(), n * nr);
+    else
+      LAKER_HIP_CHECK(hipMemcpyAsync(V.ptr, U.ptr, (size_t)n * nr * 8, hipMemcpyDeviceToDevice, st));
+    LAKER_HIP_CHECK(hipGetLastError());
+    return set_identity(Racc.as<double>(), nr, 1.0, st);
+  };
+  LAKER_TRY(load_v());
+  int rc = cholqr_pass(V.as<double>(), n, nr, 0.0, G.as<double>(), Linv.as<double>(), Racc.as<double>(),
+                       tmp.as<double>(), Vtmp.as<double>(), infob.as<int>(), &pc->ws, st);
+  if (rc == LAKER_ENUMERIC) {
+    // shifted Cholesky-QR: shift = 11 (n nr + nr (nr + 1)) u ||U||_F^2, ||U||_F^2 <= nr
+    LAKER_TRY(load_v());
+    const double u = 1.1102230246251565e-16;
+    const double shift = 11.0 * ((double)n * nr + (double)nr * (nr + 1)) * u * (double)nr;
+    rc = cholqr_pass(V.as<double>(), n, nr, shift, G.as<double>(), Linv.as<double>(), Racc.as<double>(),
+                     tmp.as<double>(), Vtmp.as<double>(), infob.as<int>(), &pc->ws, st);
+    if (rc == LAKER_OK)
+      rc = cholqr_pass(V.as<double>(), n, nr, 0.0, G.as<double>(), Linv.as<double>(), Racc.as<double>(),
+                       tmp.as<double>(), Vtmp.as<double>(), infob.as<int>(), &pc->ws, st);
+  }
+  if (rc == LAKER_OK)
+    rc = cholqr_pass(V.as<double>(), n, nr, 0.0, G.as<double>(), Linv.as<double>(), Racc.as<double>(),
+                     tmp.as<double>(), Vtmp.as<double>(), infob.as<int>(), &pc->ws, st);
+  if (rc == LAKER_ENUMERIC) return fail(LAKER_ENUMERIC, "CCCP build: Cholesky-QR of the probe responses failed");
+  LAKER_TRY(rc);
+  U.release();
+  Vtmp.release();
+  LAKER_TRY(pc->Q.ensure((size_t)n * nr * esz));
+  if (pc->dtype == LAKER_F32)
+    hipLaunchKernelGGL((convert_kernel<double, float>), dim3((unsigned)ceil_div(n * nr, 256)), dim3(256), 0,
+                       st, V.as<double>(), pc->Q.as<float>(), n * nr);
+  else
+    LAKER_HIP_CHECK(hipMemcpyAsync(pc->Q.ptr, V.ptr, (size_t)n * nr * 8, hipMemcpyDeviceToDevice, st));
+  LAKER_HIP_CHECK(hipGetLastError());
+  V.release();
+  LAKER_TRY(pc->R.ensure(nn * 8));
+  LAKER_HIP_CHECK(hipMemcpyAsync(pc->R.ptr, Racc.ptr, nn * 8, hipMemcpyDeviceToDevice, st));
+  double* Rq = pc->R.as<double>();
+
+  // 4. CCCP fixed point (FP64)
+  double rho;
+  if (nr >= n) {
+    rho = base_rho;
+  } else {
+    const double ratio = (double)nr / (double)n;
+    rho = base_rho + (1.0 - base_rho) * (1.0 - ratio) * std::min(1.0, gamma * 10.0);
+    rho = std::min(rho, 0.5);
+  }
+  pc->rho = rho;
+  const double s = 1.0 + gamma / (double)n;
+  const double a_rho = (1.0 - rho) * gamma / s + rho;
+  DevBuf Cb, Cn, Mb, Zb, Rw, Hb, scb, partb;
+  LAKER_TRY(Cb.ensure(nn * 8));
+  LAKER_TRY(Cn.ensure(nn * 8));
+  LAKER_TRY(Mb.ensure(nn * 8));
+  LAKER_TRY(Zb.ensure(nn * 8));
+  LAKER_TRY(Rw.ensure(nn * 8));
+  LAKER_TRY(Hb.ensure(nn * 8));
+  LAKER_TRY(scb.ensure(sizeof(CccpScalars)));
+  const int upd_blocks = (int)std::min<int64_t>(256, ceil_div(nn, 256));
+  LAKER_TRY(partb.ensure((size_t)upd_blocks * 2 * 8));
+  LAKER_HIP_CHECK(hipMemsetAsync(Cb.ptr, 0, nn * 8, st));
+  double a = 1.0;
+  int it_done = 0;
+  const double n_over_nr = (double)n / (double)nr;
+  for (int it = 0; it < max_iter; ++it) {
+    hipLaunchKernelGGL(form_m_kernel, dim3((unsigned)ceil_div(nn, 256)), dim3(256), 0, st, Cb.as<double>(),
+                       Mb.as<double>(), (int64_t)nr, a);
+    LAKER_HIP_CHECK(hipMemsetAsync(infob.ptr, 0, sizeof(int), st));
+    LAKER_TRY(potrf_lower(Mb.as<double>(), nr, nr, infob.as<int>(), &pc->ws, st));
+    LAKER_HIP_CHECK(hipMemcpyAsync(Zb.ptr, Rq, nn * 8, hipMemcpyDeviceToDevice, st));
+    LAKER_TRY(trsm_lower_left(Mb.as<double>(), nr, nr, Zb.as<double>(), nr, nr, &pc->ws, st));
+    hipLaunchKernelGGL(cccp_weights_kernel, dim3(nr), dim3(256), 0, st, Zb.as<double>(), Rq, Rw.as<double>(),
+                       nr, n_over_nr, epsilon);
+    LAKER_TRY((gemm<double, double, double>(false, true, nr, nr, nr, 1.0, Rw.as<double>(), nr, Rq, nr, 0.0,
+                                             Hb.as<double>(), nr, nullptr, st)));
+    hipLaunchKernelGGL(cccp_trace_kernel, dim3(1), dim3(256), 0, st, Hb.as<double>(), nr, gamma, s, rho,
+                       a_rho, n, scb.as<CccpScalars>());
+    hipLaunchKernelGGL(cccp_update_kernel, dim3(upd_blocks), dim3(256), 0, st, Hb.as<double>(),
+                       Cb.as<double>(), Cn.as<double>(), nr, gamma, s, rho, scb.as<CccpScalars>(),
+                       partb.as<double>());
+    hipLaunchKernelGGL(cccp_norms_final_kernel, dim3(1), dim3(64), 0, st, partb.as<double>(), upd_blocks,
+                       scb.as<CccpScalars>());
+    LAKER_HIP_CHECK(hipGetLastError());
+    CccpScalars hs;
+    int hinfo = 0;
+    LAKER_HIP_CHECK(hipMemcpyAsync(&hinfo, infob.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
+    LAKER_TRY(read_scalar(scb.ptr, &hs, sizeof(hs), st));
+    if (hinfo != 0) return fail(LAKER_ENUMERIC, "CCCP build: M = aI + C is not positive definite");
+    const double a_prev = a;
+    a = hs.a_new;
+    std::swap(Cb, Cn);
+    const double iso_rel = std::fabs(a - a_prev) / (std::fabs(a_prev) + 1e-12);
+    const double q_rel = std::sqrt(hs.diff2) / (std::sqrt(hs.c2) + 1e-12);
+    it_done = it + 1;
+    if (std::max(iso_rel, q_rel) < tol) break;
+  }
+  pc->a = a;
+  pc->inv_sqrt_a = 1.0 / std::sqrt(a);
+  LAKER_TRY(pc->C.ensure(nn * 8));
+  LAKER_HIP_CHECK(hipMemcpyAsync(pc->C.ptr, Cb.ptr, nn * 8, hipMemcpyDeviceToDevice, st));
+
+  // 5. S = M^{-1/2} - a^{-1/2} I by coupled Newton-Schulz on M / c
+  hipLaunchKernelGGL(form_m_kernel, dim3((unsigned)ceil_div(nn, 256)), dim3(256), 0, st, Cb.as<double>(),
+                     Mb.as<double>(), (int64_t)nr, a);
+  hipLaunchKernelGGL(frob2_kernel, dim3(1), dim3(1024), 0, st, Mb.as<double>(), nn, scb.as<double>());
+  LAKER_HIP_CHECK(hipGetLastError());
+  double fro2 = 0.0;
+  LAKER_TRY(read_scalar(scb.ptr, &fro2, 8, st));
+  const double cn = std::sqrt(fro2);
+  // Y = M / c  (in Mb), Z = I (in Zb)
+  hipLaunchKernelGGL(scale_sub_diag_kernel, dim3((unsigned)ceil_div(nn, 256)), dim3(256), 0, st,
+                     Mb.as<double>(), Mb.as<double>(), (int64_t)nr, 1.0 / cn, 0.0);
+  LAKER_TRY(set_identity(Zb.as<double>(), nr, 1.0, st));
+  const int ns_blocks = (int)std::min<int64_t>(256, ceil_div(nn, 256));
+  LAKER_TRY(partb.ensure((size_t)std::max(ns_blocks, upd_blocks) * 2 * 8));
+  double prev_err = 1e300;
+  bool ns_ok = false;
+  for (int it = 0; it < 200; ++it) {
+    // P = Z Y -> Hb ; T -> Rw ; Y <- Y T (Cn) ; Z <- T Z (Linv)
+    LAKER_TRY((gemm<double, double, double>(false, false, nr, nr, nr, 1.0, Zb.as<double>(), nr,
+                                             Mb.as<double>(), nr, 0.0, Hb.as<double>(), nr, nullptr, st)));
+    hipLaunchKernelGGL(ns_t_kernel, dim3(ns_blocks), dim3(256), 0, st, Hb.as<double>(), Rw.as<double>(),
+                       (int64_t)nr, partb.as<double>());
+    hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(64), 0, st, partb.as<double>(), ns_blocks,
+                       scb.as<double>());
+    LAKER_HIP_CHECK(hipGetLastError());
+    double err2 = 0.0;
+    LAKER_TRY(read_scalar(scb.ptr, &err2, 8, st));
+    const double err = std::sqrt(err2);
+    if (!(err == err)) break;  // NaN
+    if (err < 1e-13 * std::sqrt((double)nr) || (err > 0.5 * prev_err && err < 1e-10)) {
+      ns_ok = true;
+      break;
+    }
+    prev_err = err;
+    LAKER_TRY((gemm<double, double, double>(false, false, nr, nr, nr, 1.0, Mb.as<double>(), nr,
+                                             Rw.as<double>(), nr, 0.0, Cn.as<double>(), nr, nullptr, st)));
+    LAKER_TRY((gemm<double, double, double>(false, false, nr, nr, nr, 1.0, Rw.as<double>(), nr,
+                                             Zb.as<double>(), nr, 0.0, Linv.as<double>(), nr, nullptr, st)));
+    LAKER_HIP_CHECK(hipMemcpyAsync(Mb.ptr, Cn.ptr, nn * 8, hipMemcpyDeviceToDevice, st));
+    LAKER_HIP_CHECK(hipMemcpyAsync(Zb.ptr, Linv.ptr, nn * 8, hipMemcpyDeviceToDevice, st));
+  }
+  if (!ns_ok) return fail(LAKER_ENUMERIC, "CCCP build: Newton-Schulz inverse square root did not converge");
+  LAKER_TRY(pc->S.ensure(nn * 8));
+  // S = (Z + Z^T)/(2 sqrt(c)) - a^{-1/2} I
+  hipLaunchKernelGGL(sym_scale_kernel, dim3((unsigned)ceil_div(nn, 256)), dim3(256), 0, st, Zb.as<double>(),
+                     pc->S.as<double>(), (int64_t)nr, 1.0 / std::sqrt(cn), pc->inv_sqrt_a);
+  LAKER_HIP_CHECK(hipGetLastError());
+  LAKER_HIP_CHECK(hipStreamSynchronize(st));
+  if (iterations) *iterations = it_done;
+  return LAKER_OK;
+}
+
+
+}  // namespace laker
+
+using namespace laker;
+
+extern "C" {
+
+int laker_precond_build(laker_operator_t oh, int32_t nr, double gamma, double epsilon,
+                        double base_rho, int32_t max_iter, double tol, const void* probes,
+                        uint64_t seed, int32_t probe_precision, hipStream_t st,
+                        laker_precond_t* out, int32_t* iterations) {
+  LAKER_CHECK_ARG(oh != nullptr && out != nullptr, "operator/out handle is NULL");
+  Operator& op = *reinterpret_cast<Operator*>(oh);
+  const int64_t n = op.n;
+  LAKER_CHECK_ARG(nr >= 1 && nr <= n, "num_probes must be in [1, n]");
+  LAKER_CHECK_ARG(nr <= 4096, "num_probes > 4096 not supported");
+  LAKER_CHECK_ARG(max_iter >= 1, "max_iter must be >= 1");
+  std::unique_ptr<Precond> pc(new Precond());
+  pc->n = n;
+  pc->nr = nr;
+  pc->dtype = op.dtype;
+  const size_t esz = op.dtype == LAKER_F32 ? 4 : 8;
+  const int64_t nn = (int64_t)nr * nr;
+
+  // 1. probes and responses U = A R
+  DevBuf probe_buf, U;
+  const void* Rp = probes;
+  if (!Rp) {
+    LAKER_TRY(probe_buf.ensure((size_t)n * nr * esz));
+    if (op.dtype == LAKER_F32)
+      hipLaunchKernelGGL((randn_kernel<float>), dim3((unsigned)ceil_div(n * nr, 256)), dim3(256), 0, st,
+                         probe_buf.as<float>(), n * nr, seed);
+    else
+      hipLaunchKernelGGL((randn_kernel<double>), dim3((unsigned)ceil_div(n * nr, 256)), dim3(256), 0, st,
+                         probe_buf.as<double>(), n * nr, seed);
+    LAKER_HIP_CHECK(hipGetLastError());
+    Rp = probe_buf.ptr;
+  }
+  LAKER_TRY(U.ensure((size_t)n * nr * esz));
+  const bool clamp = op.clamp_for(LAKER_CLAMP_AUTO);
+  if (op.dtype == LAKER_F32) {
+    int prec = probe_precision;
+    if (prec == LAKER_PREC_FP64) prec = LAKER_PREC_FP32;
+    LAKER_TRY(fused_product_f32(op, prec, op.rows(), op.cols(), (const float*)Rp, nr, nr, U.as<float>(),
+                                nr, op.lambda_reg, (const float*)Rp, nr, clamp, false, 0, st));
+  } else {
+    LAKER_TRY(fused_product_f64<double>(op, op.rows(), op.cols(), (const double*)Rp, nr, nr,
+                                        U.as<double>(), nr, op.lambda_reg, (const double*)Rp, nr, clamp,
+                                        nullptr, st));
+  }
+  probe_buf.release();
+
+  int its = 0;
+  LAKER_TRY(build_from_u(op, pc, U, nr, gamma, epsilon, base_rho, max_iter, tol, st, &its));
+  if (iterations) *iterations = its;
+  *out = reinterpret_cast<laker_precond_t>(pc.release());
+  return LAKER_OK;
+}
+
+int laker_precond_build_from_responses(const void* U_in, int64_t n, int32_t nr, int32_t dtype,
+                                       double gamma, double epsilon, double base_rho, int32_t max_iter,
+                                       double tol, hipStream_t st, laker_precond_t* out,
+                                       int32_t* iterations) {
+  LAKER_CHECK_ARG(U_in != nullptr && out != nullptr, "responses/out pointer is NULL");
+  LAKER_CHECK_ARG(n >= 1 && nr >= 1 && nr <= n && nr <= 4096, "bad n / num_probes");
+  LAKER_CHECK_ARG(dtype == LAKER_F32 || dtype == LAKER_F64, "dtype must be LAKER_F32 or LAKER_F64");
+  std::unique_ptr<Precond> pc(new Precond());
+  pc->n = n;
+  pc->nr = nr;
+  pc->dtype = dtype;
+  const size_t esz = dtype == LAKER_F32 ? 4 : 8;
+  DevBuf U;
+  LAKER_TRY(U.ensure((size_t)n * nr * esz));
+  LAKER_HIP_CHECK(hipMemcpyAsync(U.ptr, U_in, (size_t)n * nr * esz, hipMemcpyDeviceToDevice, st));
+  Operator dummy;
+  int its = 0;
+  LAKER_TRY(build_from_u(dummy, pc, U, nr, gamma, epsilon, base_rho, max_iter, tol, st, &its));
+  if (iterations) *iterations = its;
+  *out = reinterpret_cast<laker_precond_t>(pc.release());
+  return LAKER_OK;
+}
+
+int laker_precond_copy_out(laker_precond_t h, int32_t which, void* dst, hipStream_t st) {
+  LAKER_CHECK_ARG(h != nullptr && dst != nullptr, "bad copy_out arguments");
+  Precond* pc = reinterpret_cast<Precond*>(h);
+  const int64_t nn = (int64_t)pc->nr * pc->nr;
+  const void* src = nullptr;
+  size_t bytes = 0;
+  switch (which) {
+    case 0: src = pc->Q.ptr; bytes = (size_t)pc->n * pc->nr * (pc->dtype == LAKER_F32 ? 4 : 8); break;
+    case 1: src = pc->R.ptr; bytes = nn * 8; break;
+    case 2: src = pc->C.ptr; bytes = nn * 8; break;
+    case 3: src = pc->S.ptr; bytes = nn * 8; break;
+    default: return fail(LAKER_EINVAL, "which must be 0 (Q), 1 (R), 2 (C) or 3 (S)");
+  }
+  LAKER_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st));
+  return LAKER_OK;
+}
+
+int laker_precond_destroy(laker_precond_t h) {
+  delete reinterpret_cast<Precond*>(h);
+  return LAKER_OK;
+}
+
+int laker_precond_info(laker_precond_t h, double* isotropic_coef, int32_t* nr, double* rho,
+                       const void** q_basis, const double** qr_r, const double** c_matrix,
+                       const double** s_matrix) {
+  LAKER_CHECK_ARG(h != nullptr, "preconditioner handle is NULL");
+  Precond* pc = reinterpret_cast<Precond*>(h);
+  if (isotropic_coef) *isotropic_coef = pc->a;
+  if (nr) *nr = pc->nr;
+  if (rho) *rho = pc->rho;
+  if (q_basis) *q_basis = pc->Q.ptr;
+  if (qr_r) *qr_r = pc->R.as<double>();
+  if (c_matrix) *c_matrix = pc->C.as<double>();
+  if (s_matrix) *s_matrix = pc->S.as<double>();
+  return LAKER_OK;
+}
+
+int laker_precond_apply(laker_precond_t h, const void* X, int32_t k, int64_t ldx, void* Z, int64_t ldz,
+                        hipStream_t stream) {
+  LAKER_CHECK_ARG(h != nullptr, "Preconditioner has not been built. Call build() first.");
+  LAKER_CHECK_ARG(X && Z && k >= 1 && ldx >= k && ldz >= k, "bad apply arguments");
+  return precond_apply(*reinterpret_cast<Precond*>(h), X, k, ldx, Z, ldz, stream);
+}
+
+}  // extern "C"

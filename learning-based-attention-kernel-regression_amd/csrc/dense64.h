@@ -1,0 +1,25 @@
+// FP64 dense helpers (device, stream-ordered).  See dense64.hip.
+#pragma once
+#include "laker_common.h"
+
+namespace laker {
+
+// C = alpha * op(A) * op(B) + beta * C, row-major, FP64 accumulation.
+// op(A) is M x K, op(B) is K x N.  ws (optional) enables split-K.
+template <class TA, class TB, class TC>
+int gemm(bool tra, bool trb, int64_t M, int64_t N, int64_t K, double alpha, const TA* A,
+         int64_t lda, const TB* B, int64_t ldb, double beta, TC* C, int64_t ldc, DevBuf* ws,
+         hipStream_t st);
+
+// In-place lower Cholesky of the n x n matrix A (upper triangle zeroed).
+// info_dev (device int, caller-zeroed) receives index+1 of the first
+// non-positive pivot.
+int potrf_lower(double* A, int64_t lda, int64_t n, int* info_dev, DevBuf* ws, hipStream_t st);
+
+// B <- L^{-1} B  (L lower n x n, B n x m).
+int trsm_lower_left(const double* L, int64_t ldl, int64_t n, double* B, int64_t ldb, int64_t m,
+                    DevBuf* ws, hipStream_t st);
+
+int set_identity(double* A, int64_t n, double diag, hipStream_t st);
+
+}  // namespace laker

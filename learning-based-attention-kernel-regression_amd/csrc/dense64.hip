@@ -1,0 +1,321 @@
+// Small dense FP64 linear algebra for the CCCP preconditioner (N_r <= 2048)
+// and the tall-skinny products of its QR and apply:
+//   gemm      C = alpha op(A) op(B) + beta C    (row-major, split-K, FP64 accumulation)
+//   potrf     blocked right-looking Cholesky, lower, in place
+//   trsm_llnn X = L^{-1} B, blocked forward substitution, in place on B
+// These replace the torch.linalg.qr / eigh calls of laker/preconditioner.py:186-193,
+// :207-280 and laker/utils.py:109-134 in the FP64 device CCCP build.
+#include "laker_common.h"
+#include "laker_internal.h"
+#include "dense64.h"
+
+namespace laker {
+
+constexpr int GBM = 64, GBN = 64, GBK = 16;
+
+// One 64x64 tile of op(A) op(B) over the K range [k0, k1), accumulated in double.
+template <class TA, class TB, bool TRA, bool TRB>
+__device__ inline void gemm_tile(const TA* __restrict__ A, int64_t lda, const TB* __restrict__ B,
+                                 int64_t ldb, int64_t M, int64_t N, int64_t k0, int64_t k1,
+                                 int64_t m0, int64_t n0, double acc[4][4],
+                                 double (*As)[GBM + 1], double (*Bs)[GBN + 1]) {
+  const int tid = threadIdx.x;
+  const int tm = tid / 16, tn = tid % 16;
+  for (int64_t kb = k0; kb < k1; kb += GBK) {
+    __syncthreads();
+    for (int e = tid; e < GBK * GBM; e += 256) {
+      int kk, mm;
+      if (TRA) { mm = e % GBM; kk = e / GBM; } else { kk = e % GBK; mm = e / GBK; }
+      const int64_t gm = m0 + mm, gk = kb + kk;
+      double v = 0.0;
+      if (gm < M && gk < k1) v = TRA ? (double)A[gk * lda + gm] : (double)A[gm * lda + gk];
+      As[kk][mm] = v;
+    }
+    for (int e = tid; e < GBK * GBN; e += 256) {
+      int kk, nn;
+      if (TRB) { kk = e % GBK; nn = e / GBK; } else { nn = e % GBN; kk = e / GBN; }
+      const int64_t gn = n0 + nn, gk = kb + kk;
+      double v = 0.0;
+      if (gn < N && gk < k1) v = TRB ? (double)B[gn * ldb + gk] : (double)B[gk * ldb + gn];
+      Bs[kk][nn] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GBK; ++kk) {
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[kk][tm + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tn + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+    }
+  }
+}
+
+// grid: (ceil(N/64), ceil(M/64), ksplit).  ksplit == 1 writes C directly.
+template <class TA, class TB, class TC, bool TRA, bool TRB>
+__global__ __launch_bounds__(256) void gemm_kernel(int64_t M, int64_t N, int64_t K, double alpha,
+                                                   const TA* __restrict__ A, int64_t lda,
+                                                   const TB* __restrict__ B, int64_t ldb,
+                                                   double beta, TC* __restrict__ C, int64_t ldc,
+                                                   double* __restrict__ part, int64_t kchunk) {
+  __shared__ double As[GBK][GBM + 1];
+  __shared__ double Bs[GBK][GBN + 1];
+  const int64_t m0 = (int64_t)blockIdx.y * GBM, n0 = (int64_t)blockIdx.x * GBN;
+  const int64_t k0 = (int64_t)blockIdx.z * kchunk;
+  const int64_t k1 = (k0 + kchunk < K) ? k0 + kchunk : K;
+  double acc[4][4] = {};
+  gemm_tile<TA, TB, TRA, TRB>(A, lda, B, ldb, M, N, k0, k1, m0, n0, acc, As, Bs);
+  const int tm = threadIdx.x / 16, tn = threadIdx.x % 16;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t gm = m0 + tm + 16 * i, gn = n0 + tn + 16 * j;
+      if (gm < M && gn < N) {
+        if (part) {
+          part[((int64_t)blockIdx.z * M + gm) * N + gn] = acc[i][j];
+        } else {
+          double v = alpha * acc[i][j];
+          if (beta != 0.0) v += beta * (double)C[gm * ldc + gn];
+          C[gm * ldc + gn] = (TC)v;
+        }
+      }
+    }
+}
+
+template <class TC>
+__global__ void gemm_splitk_reduce(const double* __restrict__ part, int64_t M, int64_t N, int ks,
+                                   double alpha, double beta, TC* __restrict__ C, int64_t ldc) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= M * N) return;
+  const int64_t m = idx / N, n = idx % N;
+  double s = 0.0;
+  for (int z = 0; z < ks; ++z) s += part[(int64_t)z * M * N + idx];
+  double v = alpha * s;
+  if (beta != 0.0) v += beta * (double)C[m * ldc + n];
+  C[m * ldc + n] = (TC)v;
+}
+
+template <class TA, class TB, class TC>
+int gemm(bool tra, bool trb, int64_t M, int64_t N, int64_t K, double alpha, const TA* A,
+         int64_t lda, const TB* B, int64_t ldb, double beta, TC* C, int64_t ldc, DevBuf* ws,
+         hipStream_t st) {
+  if (M <= 0 || N <= 0) return LAKER_OK;
+  const int64_t tiles = ceil_div(M, GBM) * ceil_div(N, GBN);
+  int ks = 1;
+  if (ws) {
+    while (tiles * ks < 512 && K / (ks * 2) >= 256 && ks < 256) ks *= 2;
+  }
+  const int64_t kchunk = round_up(ceil_div(std::max<int64_t>(K, 1), ks), GBK);
+  ks = (int)ceil_div(std::max<int64_t>(K, 1), kchunk);
+  double* part = nullptr;
+  if (ks > 1) {
+    LAKER_TRY(ws->ensure((size_t)ks * M * N * sizeof(double)));
+    part = ws->as<double>();
+  }
+  dim3 grid((unsigned)ceil_div(N, GBN), (unsigned)ceil_div(M, GBM), (unsigned)ks), block(256);
+#define LAKER_GEMM(TRA_, TRB_)                                                                  \
+  hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, TRA_, TRB_>), grid, block, 0, st, M, N, K, alpha, A, \
+                     lda, B, ldb, beta, C, ldc, part, kchunk)
+  if (tra) {
+    if (trb) LAKER_GEMM(true, true); else LAKER_GEMM(true, false);
+  } else {
+    if (trb) LAKER_GEMM(false, true); else LAKER_GEMM(false, false);
+  }
+#undef LAKER_GEMM
+  LAKER_HIP_CHECK(hipGetLastError());
+  if (ks > 1) {
+    hipLaunchKernelGGL((gemm_splitk_reduce<TC>), dim3((unsigned)ceil_div(M * N, 256)), dim3(256), 0,
+                       st, part, M, N, ks, alpha, beta, C, ldc);
+    LAKER_HIP_CHECK(hipGetLastError());
+  }
+  return LAKER_OK;
+}
+
+#define LAKER_INST_GEMM(TA, TB, TC)                                                             \
+  template int gemm<TA, TB, TC>(bool, bool, int64_t, int64_t, int64_t, double, const TA*,     \
+                                int64_t, const TB*, int64_t, double, TC*, int64_t, DevBuf*,  \
+                                hipStream_t);
+LAKER_INST_GEMM(double, double, double)
+LAKER_INST_GEMM(float, float, double)
+LAKER_INST_GEMM(float, double, double)
+LAKER_INST_GEMM(float, double, float)
+LAKER_INST_GEMM(double, double, float)
+LAKER_INST_GEMM(float, float, float)
+LAKER_INST_GEMM(double, float, double)
+LAKER_INST_GEMM(double, float, float)
+#undef LAKER_INST_GEMM
+
+// ---------------------------------------------------------------------------
+// Cholesky (lower), blocked with NB = 64.
+// ---------------------------------------------------------------------------
+constexpr int PNB = 64;
+
+// Unblocked Cholesky of the nb x nb diagonal block at (k0,k0), in LDS.
+// Sets *info = global index + 1 of the first non-positive pivot.
+__global__ __launch_bounds__(256) void potrf_diag_kernel(double* __restrict__ A, int64_t lda,
+                                                         int64_t k0, int nb, int* __restrict__ info) {
+  __shared__ double L[PNB][PNB + 1];
+  for (int e = threadIdx.x; e < nb * nb; e += 256) {
+    const int i = e / nb, j = e % nb;
+    L[i][j] = A[(k0 + i) * lda + k0 + j];
+  }
+  __syncthreads();
+  for (int j = 0; j < nb; ++j) {
+    // L[j][j] = sqrt(A[j][j] - sum_{p<j} L[j][p]^2) -- updates are applied right-looking
+    if (threadIdx.x == 0) {
+      double v = L[j][j];
+      if (!(v > 0.0)) {
+        if (*info == 0) *info = (int)(k0 + j + 1);
+        v = 1e300;  // keep going without NaNs; caller checks info
+      }
+      L[j][j] = sqrt(v);
+    }
+    __syncthreads();
+    const double djj = L[j][j];
+    for (int i = j + 1 + threadIdx.x; i < nb; i += 256) L[i][j] /= djj;
+    __syncthreads();
+    // trailing update of the lower triangle within the block
+    const int rem = nb - j - 1;
+    for (int e = threadIdx.x; e < rem * rem; e += 256) {
+      const int i = j + 1 + e / rem, c = j + 1 + e % rem;
+      if (c <= i) L[i][c] -= L[i][j] * L[c][j];
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < nb * nb; e += 256) {
+    const int i = e / nb, j = e % nb;
+    A[(k0 + i) * lda + k0 + j] = (j <= i) ? L[i][j] : 0.0;
+  }
+}
+
+// Rows below the diagonal block: A[r, k0:k0+nb] <- A[r, k0:k0+nb] L_kk^{-T}
+// (forward substitution per row).  One thread per row.
+__global__ __launch_bounds__(256) void potrf_panel_kernel(double* __restrict__ A, int64_t lda,
+                                                          int64_t n, int64_t k0, int nb) {
+  __shared__ double L[PNB][PNB + 1];
+  for (int e = threadIdx.x; e < nb * nb; e += 256) {
+    const int i = e / nb, j = e % nb;
+    L[i][j] = A[(k0 + i) * lda + k0 + j];
+  }
+  __syncthreads();
+  const int64_t r = k0 + nb + (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (r >= n) return;
+  double x[PNB];
+#pragma unroll
+  for (int j = 0; j < PNB; ++j) x[j] = (j < nb) ? A[r * lda + k0 + j] : 0.0;
+#pragma unroll
+  for (int j = 0; j < PNB; ++j) {
+    if (j < nb) {
+      double v = x[j];
+#pragma unroll
+      for (int p = 0; p < PNB; ++p)
+        if (p < j) v -= x[p] * L[j][p];
+      x[j] = v / L[j][j];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < PNB; ++j)
+    if (j < nb) A[r * lda + k0 + j] = x[j];
+}
+
+__global__ void zero_upper_kernel(double* __restrict__ A, int64_t lda, int64_t n) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  const int64_t i = idx / n, j = idx % n;
+  if (j > i) A[i * lda + j] = 0.0;
+}
+
+int potrf_lower(double* A, int64_t lda, int64_t n, int* info_dev, DevBuf* ws, hipStream_t st) {
+  for (int64_t k0 = 0; k0 < n; k0 += PNB) {
+    const int nb = (int)std::min<int64_t>(PNB, n - k0);
+    hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, st, A, lda, k0, nb, info_dev);
+    const int64_t rem = n - k0 - nb;
+    if (rem > 0) {
+      hipLaunchKernelGGL(potrf_panel_kernel, dim3((unsigned)ceil_div(rem, 256)), dim3(256), 0, st, A,
+                         lda, n, k0, nb);
+      // A22 -= L21 L21^T  (full square of the trailing block; upper part is scratch)
+      double* L21 = A + (k0 + nb) * lda + k0;
+      double* A22 = A + (k0 + nb) * lda + k0 + nb;
+      LAKER_TRY((gemm<double, double, double>(false, true, rem, rem, nb, -1.0, L21, lda, L21, lda,
+                                               1.0, A22, lda, ws, st)));
+    }
+  }
+  hipLaunchKernelGGL(zero_upper_kernel, dim3((unsigned)ceil_div(n * n, 256)), dim3(256), 0, st, A,
+                     lda, n);
+  LAKER_HIP_CHECK(hipGetLastError());
+  return LAKER_OK;
+}
+
+// ---------------------------------------------------------------------------
+// X = L^{-1} B in place (B: n x m, row-major ldb), L lower n x n.
+// ---------------------------------------------------------------------------
+// Diagonal-block solve: B[k0:k0+nb, c] <- L_kk^{-1} B[k0:k0+nb, c], one thread per column.
+__global__ __launch_bounds__(256) void trsm_diag_kernel(const double* __restrict__ L, int64_t ldl,
+                                                        double* __restrict__ B, int64_t ldb,
+                                                        int64_t m, int64_t k0, int nb) {
+  __shared__ double Ls[PNB][PNB + 1];
+  for (int e = threadIdx.x; e < nb * nb; e += 256) {
+    const int i = e / nb, j = e % nb;
+    Ls[i][j] = L[(k0 + i) * ldl + k0 + j];
+  }
+  __syncthreads();
+  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= m) return;
+  double x[PNB];
+#pragma unroll
+  for (int i = 0; i < PNB; ++i) x[i] = (i < nb) ? B[(k0 + i) * ldb + c] : 0.0;
+#pragma unroll
+  for (int i = 0; i < PNB; ++i) {
+    if (i < nb) {
+      double v = x[i];
+#pragma unroll
+      for (int p = 0; p < PNB; ++p)
+        if (p < i) v -= Ls[i][p] * x[p];
+      x[i] = v / Ls[i][i];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < PNB; ++i)
+    if (i < nb) B[(k0 + i) * ldb + c] = x[i];
+}
+
+int trsm_lower_left(const double* L, int64_t ldl, int64_t n, double* B, int64_t ldb, int64_t m,
+                    DevBuf* ws, hipStream_t st) {
+  for (int64_t k0 = 0; k0 < n; k0 += PNB) {
+    const int nb = (int)std::min<int64_t>(PNB, n - k0);
+    hipLaunchKernelGGL(trsm_diag_kernel, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, st, L, ldl,
+                       B, ldb, m, k0, nb);
+    const int64_t rem = n - k0 - nb;
+    if (rem > 0) {
+      // B[k0+nb:, :] -= L[k0+nb:, k0:k0+nb] * B[k0:k0+nb, :]
+      LAKER_TRY((gemm<double, double, double>(false, false, rem, m, nb, -1.0,
+                                               L + (k0 + nb) * ldl + k0, ldl, B + k0 * ldb, ldb, 1.0,
+                                               B + (k0 + nb) * ldb, ldb, ws, st)));
+    }
+  }
+  LAKER_HIP_CHECK(hipGetLastError());
+  return LAKER_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise helpers
+// ---------------------------------------------------------------------------
+__global__ void set_identity_kernel(double* __restrict__ A, int64_t n, double diag) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  A[idx] = (idx / n == idx % n) ? diag : 0.0;
+}
+
+int set_identity(double* A, int64_t n, double diag, hipStream_t st) {
+  hipLaunchKernelGGL(set_identity_kernel, dim3((unsigned)ceil_div(n * n, 256)), dim3(256), 0, st, A,
+                     n, diag);
+  LAKER_HIP_CHECK(hipGetLastError());
+  return LAKER_OK;
+}
+
+}  // namespace laker

@@ -1,0 +1,83 @@
+// Internal types shared by the translation units of the LAKER HIP library.
+#pragma once
+#include <mutex>
+
+#include "laker_common.h"
+
+namespace laker {
+
+// Non-owning view of one side (rows or columns) of a fused product.
+struct Side {
+  int64_t n = 0;
+  int64_t npad = 0;                // rows padded to a multiple of 64, zero-filled
+  const void* pack32 = nullptr;    // FP32 MFMA operand rows (ka32 floats)
+  const void* pack16 = nullptr;    // BF16 hi/lo MFMA operand rows (ka16 bf16)
+  const double* e64 = nullptr;     // FP64 copy, n x d
+};
+
+// Owning storage of a packed embedding set.
+struct SideStore {
+  int64_t n = 0, npad = 0;
+  DevBuf col32, row32, col16, row16, e64, user;
+};
+
+struct Operator {
+  int64_t n = 0;
+  int d = 0;
+  int dtype = LAKER_F32;
+  double lambda_reg = 0.0;
+  double scale = 1.0;
+  int clamp_policy = LAKER_CLAMP_AUTO;
+  double clamp_limit = 80.0;
+  double max_sq_norm = 0.0;
+  bool skip_clamp = true;
+  int ka32 = 0, ka16 = 0;
+  SideStore self;
+  SideStore query;  // packed query set of the last cross product
+  std::mutex query_mu;
+  DevBuf part;      // split-K partial sums
+  DevBuf scratch;   // backward right-hand sides
+  DevBuf pcg_ws;    // PCG vectors, partial sums and device scalars
+  DevBuf mixed_ws;  // refinement residual and FP32 correction vectors
+
+  int ensure_part(size_t bytes) { return part.ensure(bytes); }
+  bool clamp_for(int policy) const {
+    if (policy == LAKER_CLAMP_ALWAYS) return true;
+    if (policy == LAKER_CLAMP_NEVER) return false;
+    if (clamp_policy == LAKER_CLAMP_ALWAYS) return true;
+    if (clamp_policy == LAKER_CLAMP_NEVER) return false;
+    return !skip_clamp;
+  }
+  Side rows() const { return view(self, true); }
+  Side cols() const { return view(self, false); }
+  static Side view(const SideStore& s, bool as_rows) {
+    Side v;
+    v.n = s.n;
+    v.npad = s.npad;
+    v.pack32 = as_rows ? s.row32.ptr : s.col32.ptr;
+    v.pack16 = as_rows ? s.row16.ptr : s.col16.ptr;
+    v.e64 = s.e64.as<double>();
+    return v;
+  }
+  const float* e_user32() const { return self.user.as<float>(); }
+};
+
+// Pack an embedding set (row side pre-scaled by scale*log2(e)).
+int pack_store(SideStore& s, const void* E, int dtype, int64_t n, int d, int ka32, int ka16,
+               double scale, bool need_col, bool need_row, hipStream_t st);
+
+// Y (rows.n x k) = [lam * Xr] + exp(min(s*Erow.Ecol^T, c)) X   (FP32 vectors)
+// deriv: use the backward weights (0 where the clamp is active).
+// accumulate: add into Y instead of overwriting.
+int fused_product_f32(Operator& op, int precision, const Side& rows, const Side& cols,
+                      const float* X, int k, int64_t ldx, float* Y, int64_t ldy, double lam,
+                      const float* Xr, int64_t ldxr, bool clamp, bool deriv, int accumulate,
+                      hipStream_t st);
+
+// FP64 product; when B != nullptr writes the residual Y = B - (lam Xr + G X).
+template <class XT>
+int fused_product_f64(Operator& op, const Side& rows, const Side& cols, const XT* X, int k,
+                      int64_t ldx, double* Y, int64_t ldy, double lam, const XT* Xr, int64_t ldxr,
+                      bool clamp, const double* B, hipStream_t st);
+
+}  // namespace laker

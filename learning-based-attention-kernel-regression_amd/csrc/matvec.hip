@@ -1,0 +1,936 @@
+// Fused attention-kernel matvec  Y = lambda X + exp(min(s E E^T, c)) X  on gfx950.
+//
+// The n x n kernel is never stored.  A wave owns ITW row tiles of 32 rows and
+// streams 32-row column tiles J of E:
+//   1. Gram tile on MFMA, computed transposed (D[jj][ii] = e_J[jj] . e_I[ii]) so
+//      that each lane owns one output row ii = lane%32 and 16 column indices
+//      jj(v) = 8*(v/4) + 4*(lane/32) + v%4 (the 32x32 MFMA output layout);
+//   2. exp2 in registers (the row-side operand is pre-scaled by s*log2(e), so
+//      the MFMA output is already the exp2 argument);
+//   3. the G.X reduction: VALU FMAs for k <= 4 columns, MFMA (FP32 or BF16)
+//      with the exp tile fed straight back as the A operand for larger k.
+// Column tiles are split over `jsplit` partial sums (jsplit = 8 maps one split
+// to one XCD under round-robin dispatch), reduced deterministically by a
+// second kernel that also adds lambda X.
+//
+// Reference behaviour restated: laker/kernels.py:80-130 (exp_safe clamp,
+// dense matvec), :154-185 (skip-clamp predicate), :187-270 (matvec).
+
+#include "laker_common.h"
+#include "laker_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <mutex>
+
+namespace laker {
+
+// ---------------------------------------------------------------------------
+// Packing.  Row-side (B operand) arrays are pre-scaled by c = s*log2(e).
+//   F32:   ka = round_up(d,2) floats per row; MFMA step s contracts dims
+//          {s, s + ka/2} (lane half h takes dim s + h*ka/2).
+//   BF16x4: ka = round_up(4d,16) bf16 per row: col side [hi|hi|lo|lo],
+//          row side [hi'|lo'|hi'|lo'] so that the K=4d contraction equals
+//          (hi+lo).(hi'+lo') exactly in FP32 accumulation.
+// Rows are padded with zeros to a multiple of 32.
+// ---------------------------------------------------------------------------
+
+template <class TE>
+__global__ void pack_f32_kernel(const TE* __restrict__ E, int64_t n, int d, int64_t npad, int ka,
+                                float cscale, float* __restrict__ colside, float* __restrict__ rowside) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= npad * ka) return;
+  int64_t i = idx / ka;
+  int p = (int)(idx % ka);
+  // natural order: lane half h reads positions [h*ka/2, (h+1)*ka/2), so MFMA
+  // step s contracts dims s and s + ka/2.
+  float v = 0.f;
+  if (i < n && p < d) v = (float)E[i * d + p];
+  if (colside) colside[idx] = v;
+  if (rowside) rowside[idx] = v * cscale;
+}
+
+template <class TE>
+__global__ void pack_bf16x4_kernel(const TE* __restrict__ E, int64_t n, int d, int64_t npad, int ka,
+                                   float cscale, uint16_t* __restrict__ colside,
+                                   uint16_t* __restrict__ rowside) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= npad * ka) return;
+  int64_t i = idx / ka;
+  int p = (int)(idx % ka);
+  uint16_t cv = 0, rv = 0;
+  if (i < n && p < 4 * d) {
+    int seg = p / d, dim = p % d;
+    float e = (float)E[i * d + dim];
+    uint16_t hi = f32_to_bf16_bits(e);
+    uint16_t lo = f32_to_bf16_bits(e - bf16_bits_to_f32(hi));
+    float es = e * cscale;
+    uint16_t his = f32_to_bf16_bits(es);
+    uint16_t los = f32_to_bf16_bits(es - bf16_bits_to_f32(his));
+    // col side: hi, hi, lo, lo ; row side: hi', lo', hi', lo'
+    cv = (seg < 2) ? hi : lo;
+    rv = (seg & 1) ? los : his;
+  }
+  if (colside) colside[idx] = cv;
+  if (rowside) rowside[idx] = rv;
+}
+
+template <class TE>
+__global__ void pack_f64_kernel(const TE* __restrict__ E, int64_t n, int d, double* __restrict__ out) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * d) return;
+  out[idx] = (double)E[idx];
+}
+
+// max_i ||e_i||^2 (FP64) -> out[0] via atomicMax on the bit pattern (non-negative doubles).
+template <class TE>
+__global__ void max_sqnorm_kernel(const TE* __restrict__ E, int64_t n, int d,
+                                  unsigned long long* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double s = 0.0;
+  if (i < n)
+    for (int k = 0; k < d; ++k) {
+      double v = (double)E[i * d + k];
+      s += v * v;
+    }
+  for (int off = 32; off > 0; off >>= 1) s = fmax(s, __shfl_xor(s, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__builtin_bit_cast(uint64_t, s));
+}
+
+// ---------------------------------------------------------------------------
+// Gram tile helpers
+// ---------------------------------------------------------------------------
+
+template <int MODE>
+struct GramOps;
+
+// FP32 MFMA 32x32x2: ka floats per row, steps = ka/2.
+template <>
+struct GramOps<LAKER_PREC_FP32> {
+  template <int NS>
+  struct Operand {
+    float v[NS];
+  };
+  template <int NS>
+  __device__ static inline void load(Operand<NS>& op, const void* base, int64_t row, int ka, int h) {
+    const float* p = reinterpret_cast<const float*>(base) + row * ka + h * (ka / 2);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) op.v[s] = (s < ka / 2) ? p[s] : 0.f;
+  }
+  template <int NS>
+  __device__ static inline floatx16 gram(const Operand<NS>& a, const Operand<NS>& b, int ka) {
+    floatx16 acc = {0};
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      if (s < ka / 2) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.v[s], b.v[s], acc, 0, 0, 0);
+    return acc;
+  }
+};
+
+// BF16 MFMA 32x32x16 on the hi/lo packed operands: chunks = ka/16.
+template <>
+struct GramOps<LAKER_PREC_MIXED> {
+  template <int NS>
+  struct Operand {
+    bf16x8 v[NS];
+  };
+  template <int NS>
+  __device__ static inline void load(Operand<NS>& op, const void* base, int64_t row, int ka, int h) {
+    const uint16_t* p = reinterpret_cast<const uint16_t*>(base) + row * ka + h * 8;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      if (q < ka / 16) {
+        shortx8 raw = *reinterpret_cast<const shortx8*>(p + q * 16);
+        op.v[q] = __builtin_bit_cast(bf16x8, raw);
+      } else {
+        op.v[q] = __builtin_bit_cast(bf16x8, shortx8{0, 0, 0, 0, 0, 0, 0, 0});
+      }
+    }
+  }
+  template <int NS>
+  __device__ static inline floatx16 gram(const Operand<NS>& a, const Operand<NS>& b, int ka) {
+    floatx16 acc = {0};
+#pragma unroll
+    for (int q = 0; q < NS; ++q)
+      if (q < ka / 16) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.v[q], b.v[q], acc, 0, 0, 0);
+    return acc;
+  }
+};
+
+__device__ inline int jj_of(int v, int h) { return 8 * (v >> 2) + 4 * h + (v & 3); }
+
+// Element transform of the exp2 argument.  DERIV: weight of the backward
+// (0 where the clamp is active, i.e. the clamped entry has zero derivative).
+template <bool CLAMP, bool DERIV>
+__device__ inline float kexp(float t, float clampv) {
+  if constexpr (DERIV) {
+    if (CLAMP && t > clampv) return 0.f;
+    return __builtin_amdgcn_exp2f(t);
+  } else {
+    if constexpr (CLAMP) t = fminf(t, clampv);
+    return __builtin_amdgcn_exp2f(t);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Vector kernel: k <= KC columns accumulated on VALU.
+// grid.x = n_row_blocks * jsplit; block = 256 (4 waves, each ITW row tiles).
+// part layout: part[js][i][c], i < n_rows, c < KC.
+// ---------------------------------------------------------------------------
+template <int MODE, int NS, int KC, int ITW, bool CLAMP, bool DERIV>
+__global__ __launch_bounds__(256) void mv_vec_kernel(
+    const void* __restrict__ colpack, const void* __restrict__ rowpack, int ka, int64_t n_rows,
+    int64_t n_cols, const float* __restrict__ X, int64_t ldx, int kcols,
+    float* __restrict__ part, int jsplit, float clampv) {
+  using G = GramOps<MODE>;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, h = lane >> 5, li = lane & 31;
+  const int js = blockIdx.x % jsplit;
+  const int64_t ib = blockIdx.x / jsplit;
+  const int64_t I0 = (ib * 4 + w) * (32 * ITW);
+  if (I0 >= n_rows) return;
+
+  typename G::template Operand<NS> B[ITW];
+#pragma unroll
+  for (int it = 0; it < ITW; ++it) G::load(B[it], rowpack, I0 + it * 32 + li, ka, h);
+
+  const int64_t ntiles = (n_cols + 31) / 32;
+  const int64_t t0 = ntiles * js / jsplit, t1 = ntiles * (js + 1) / jsplit;
+
+  float acc[ITW][KC];
+#pragma unroll
+  for (int it = 0; it < ITW; ++it)
+#pragma unroll
+    for (int c = 0; c < KC; ++c) acc[it][c] = 0.f;
+
+  for (int64_t t = t0; t < t1; ++t) {
+    const int64_t J0 = t * 32;
+    typename G::template Operand<NS> A;
+    G::load(A, colpack, J0 + li, ka, h);
+    float xv[16][KC];
+    if (J0 + 32 <= n_cols) {
+#pragma unroll
+      for (int v = 0; v < 16; ++v)
+#pragma unroll
+        for (int c = 0; c < KC; ++c)
+          xv[v][c] = (c < kcols) ? X[(J0 + jj_of(v, h)) * ldx + c] : 0.f;
+    } else {
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int64_t j = J0 + jj_of(v, h);
+#pragma unroll
+        for (int c = 0; c < KC; ++c) xv[v][c] = (j < n_cols && c < kcols) ? X[j * ldx + c] : 0.f;
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < ITW; ++it) {
+      floatx16 D = G::gram(A, B[it], ka);
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const float e = kexp<CLAMP, DERIV>(D[v], clampv);
+#pragma unroll
+        for (int c = 0; c < KC; ++c) acc[it][c] = __builtin_fmaf(e, xv[v][c], acc[it][c]);
+      }
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < ITW; ++it) {
+#pragma unroll
+    for (int c = 0; c < KC; ++c) acc[it][c] += __shfl_xor(acc[it][c], 32, 64);
+    const int64_t i = I0 + it * 32 + li;
+    if (h == 0 && i < n_rows) {
+#pragma unroll
+      for (int c = 0; c < KC; ++c)
+        if (c < kcols) part[((int64_t)js * n_rows + i) * KC + c] = acc[it][c];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Multi-RHS kernel: G.X on MFMA.  Each wave owns ITW row tiles and NCC chunks
+// of 32 columns.  X tiles (32 rows x 32*NCC cols) are staged in LDS per
+// column tile and shared by the 4 waves of a block.
+// GX = LAKER_PREC_FP32: 16 x mfma_f32_32x32x2f32 per 32x32 block.
+// GX = LAKER_PREC_BF16: P and X rounded to bf16, 2 x mfma 32x32x16.
+// GX = LAKER_PREC_MIXED: P and X split hi/lo, 3 products (hi.hi + lo.hi + hi.lo).
+// part layout: part[js][i][c], c < ldp (= k).
+// ---------------------------------------------------------------------------
+template <int MODE, int NS, int GX, int NCC, int ITW, bool CLAMP, bool DERIV>
+__global__ __launch_bounds__(256) void mv_multi_kernel(
+    const void* __restrict__ colpack, const void* __restrict__ rowpack, int ka, int64_t n_rows,
+    int64_t n_cols, const float* __restrict__ X, int64_t ldx, int kcols, int c_base,
+    float* __restrict__ part, int64_t ldp, int jsplit, float clampv) {
+  using G = GramOps<MODE>;
+  constexpr int KW = 32 * NCC;
+  __shared__ float xs[32][KW + 1];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, h = lane >> 5, li = lane & 31;
+  const int js = blockIdx.x % jsplit;
+  const int64_t ib = blockIdx.x / jsplit;
+  const int64_t I0 = (ib * 4 + w) * (32 * ITW);
+  const bool active = I0 < n_rows;
+
+  typename G::template Operand<NS> B[ITW];
+  if (active) {
+#pragma unroll
+    for (int it = 0; it < ITW; ++it) G::load(B[it], rowpack, I0 + it * 32 + li, ka, h);
+  }
+  const int64_t ntiles = (n_cols + 31) / 32;
+  const int64_t t0 = ntiles * js / jsplit, t1 = ntiles * (js + 1) / jsplit;
+
+  floatx16 acc[ITW][NCC];
+#pragma unroll
+  for (int it = 0; it < ITW; ++it)
+#pragma unroll
+    for (int q = 0; q < NCC; ++q) acc[it][q] = floatx16{0};
+
+  for (int64_t t = t0; t < t1; ++t) {
+    const int64_t J0 = t * 32;
+    __syncthreads();
+    for (int e = threadIdx.x; e < 32 * KW; e += 256) {
+      const int r = e / KW, c = e % KW;
+      const int64_t j = J0 + r;
+      const int cc = c_base + c;
+      xs[r][c] = (j < n_cols && cc < kcols) ? X[j * ldx + cc] : 0.f;
+    }
+    __syncthreads();
+    if (!active) continue;
+    typename G::template Operand<NS> A;
+    G::load(A, colpack, J0 + li, ka, h);
+#pragma unroll
+    for (int it = 0; it < ITW; ++it) {
+      floatx16 D = G::gram(A, B[it], ka);
+      float P[16];
+#pragma unroll
+      for (int v = 0; v < 16; ++v) P[v] = kexp<CLAMP, DERIV>(D[v], clampv);
+      if constexpr (GX == LAKER_PREC_FP32) {
+#pragma unroll
+        for (int q = 0; q < NCC; ++q)
+#pragma unroll
+          for (int v = 0; v < 16; ++v)
+            acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x2f32(P[v], xs[jj_of(v, h)][q * 32 + li],
+                                                              acc[it][q], 0, 0, 0);
+      } else {
+        // K ordering of chunk u (u=0,1): k = 8h + t  <->  j = jj(8u + t, h)
+        bf16x8 Ph[2], Pl[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+          for (int t2 = 0; t2 < 8; ++t2) {
+            const float p = P[8 * u + t2];
+            const uint16_t hb = f32_to_bf16_bits(p);
+            Ph[u][t2] = __builtin_bit_cast(__bf16, hb);
+            if constexpr (GX == LAKER_PREC_MIXED)
+              Pl[u][t2] = __builtin_bit_cast(__bf16, f32_to_bf16_bits(p - bf16_bits_to_f32(hb)));
+          }
+#pragma unroll
+        for (int q = 0; q < NCC; ++q) {
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            bf16x8 Xh, Xl;
+#pragma unroll
+            for (int t2 = 0; t2 < 8; ++t2) {
+              const float xvv = xs[jj_of(8 * u + t2, h)][q * 32 + li];
+              const uint16_t hb = f32_to_bf16_bits(xvv);
+              Xh[t2] = __builtin_bit_cast(__bf16, hb);
+              if constexpr (GX == LAKER_PREC_MIXED)
+                Xl[t2] = __builtin_bit_cast(__bf16, f32_to_bf16_bits(xvv - bf16_bits_to_f32(hb)));
+            }
+            acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ph[u], Xh, acc[it][q], 0, 0, 0);
+            if constexpr (GX == LAKER_PREC_MIXED) {
+              acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Pl[u], Xh, acc[it][q], 0, 0, 0);
+              acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ph[u], Xl, acc[it][q], 0, 0, 0);
+            }
+          }
+        }
+      }
+    }
+  }
+  if (!active) return;
+  // acc[it][q][r]: row i = I0 + it*32 + 8*(r/4) + 4*h + r%4 ; column c = c_base + q*32 + li
+#pragma unroll
+  for (int it = 0; it < ITW; ++it)
+#pragma unroll
+    for (int q = 0; q < NCC; ++q)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t i = I0 + it * 32 + jj_of(r, h);
+        const int c = c_base + q * 32 + li;
+        if (i < n_rows && c < kcols) part[((int64_t)js * n_rows + i) * ldp + c] = acc[it][q][r];
+      }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 kernel: one thread per output row, column rows staged in LDS.
+// grid.x = ceil(n_rows/256) * jsplit.  XT: type of X (float or double).
+// ---------------------------------------------------------------------------
+template <int KC, int DMAX, bool CLAMP, class XT>
+__global__ __launch_bounds__(256) void mv_f64_kernel(const double* __restrict__ Ecol,
+                                                      const double* __restrict__ Erow, int d,
+                                                      int64_t n_rows, int64_t n_cols, double scale,
+                                                      double clampv, const XT* __restrict__ X,
+                                                      int64_t ldx, int kcols, int c_base,
+                                                      double* __restrict__ part, int64_t ldp,
+                                                      int jsplit) {
+  __shared__ double es[64][DMAX + 1];
+  __shared__ double xs[64][KC];
+  const int js = blockIdx.x % jsplit;
+  const int64_t ib = blockIdx.x / jsplit;
+  const int64_t i = ib * 256 + threadIdx.x;
+  double ei[DMAX];
+#pragma unroll
+  for (int q = 0; q < DMAX; ++q) ei[q] = (i < n_rows && q < d) ? Erow[i * d + q] * scale : 0.0;
+  double acc[KC];
+#pragma unroll
+  for (int c = 0; c < KC; ++c) acc[c] = 0.0;
+  const int64_t ntiles = (n_cols + 63) / 64;
+  const int64_t t0 = ntiles * js / jsplit, t1 = ntiles * (js + 1) / jsplit;
+  for (int64_t t = t0; t < t1; ++t) {
+    const int64_t J0 = t * 64;
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * DMAX; e += 256) {
+      const int r = e / DMAX, q = e % DMAX;
+      const int64_t j = J0 + r;
+      es[r][q] = (j < n_cols && q < d) ? Ecol[j * d + q] : 0.0;
+    }
+    for (int e = threadIdx.x; e < 64 * KC; e += 256) {
+      const int r = e / KC, c = e % KC;
+      const int64_t j = J0 + r;
+      xs[r][c] = (j < n_cols && c_base + c < kcols) ? (double)X[j * ldx + c_base + c] : 0.0;
+    }
+    __syncthreads();
+    const int rmax = (n_cols - J0) < 64 ? (int)(n_cols - J0) : 64;
+    for (int r = 0; r < rmax; ++r) {
+      double g = 0.0;
+#pragma unroll
+      for (int q = 0; q < DMAX; ++q) g = fma(ei[q], es[r][q], g);
+      if constexpr (CLAMP) g = fmin(g, clampv);
+      const double ev = exp(g);
+#pragma unroll
+      for (int c = 0; c < KC; ++c) acc[c] = fma(ev, xs[r][c], acc[c]);
+    }
+  }
+  if (i < n_rows) {
+#pragma unroll
+    for (int c = 0; c < KC; ++c)
+      if (c_base + c < kcols) part[((int64_t)js * n_rows + i) * ldp + c_base + c] = acc[c];
+  }
+}
+
+// Y[i][c] = lam * Xr[i][c] + sum_js part[js][i][c]     (Xr may be null)
+// MODE_RES: Y = B - lam*Xr - sum   (FP64 residual)
+template <class PT, class XT, class YT, bool RES>
+__global__ void mv_reduce_kernel(const PT* __restrict__ part, int64_t n_rows, int k, int64_t ldp,
+                                 int jsplit, double lam, const XT* __restrict__ Xr, int64_t ldx,
+                                 const YT* __restrict__ B, YT* __restrict__ Y, int64_t ldy,
+                                 int accumulate) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n_rows * k) return;
+  const int64_t i = idx / k;
+  const int c = (int)(idx % k);
+  double s = 0.0;
+  for (int js = 0; js < jsplit; ++js) s += (double)part[((int64_t)js * n_rows + i) * ldp + c];
+  if (Xr) s += lam * (double)Xr[i * ldx + c];
+  if constexpr (RES) {
+    Y[i * ldy + c] = (YT)((double)B[i * ldy + c] - s);
+  } else {
+    if (accumulate) s += (double)Y[i * ldy + c];
+    Y[i * ldy + c] = (YT)s;
+  }
+}
+
+template <class T>
+__global__ void diag_kernel(const T* __restrict__ E, int64_t n, int d, double scale, double lam,
+                            T* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int q = 0; q < d; ++q) {
+    double v = (double)E[i * d + q];
+    s += v * v;
+  }
+  out[i] = (T)(lam + exp(scale * s));
+}
+
+// K[i][j] = exp(min(s * x_i . y_j, c))  (FP32 math for FP32, FP64 for FP64)
+template <class T>
+__global__ void kernel_eval_kernel(const T* __restrict__ Xe, int64_t m, const T* __restrict__ Ye,
+                                   int64_t p, int d, double scale, int clamp, double clampv,
+                                   double diag_add, T* __restrict__ K, int64_t ldk) {
+  const int64_t total = m * p;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = idx / p, j = idx % p;
+    T g = 0;
+    for (int q = 0; q < d; ++q) g += Xe[i * d + q] * Ye[j * d + q];
+    g = g * (T)scale;
+    if (clamp) g = g < (T)clampv ? g : (T)clampv;
+    T v = exp(g);
+    if (i == j) v += (T)diag_add;
+    K[i * ldk + j] = v;
+  }
+}
+
+static dim3 eval_grid(int64_t total) {
+  return dim3((unsigned)std::min<int64_t>(ceil_div(total, 256), 65536));
+}
+
+// ---------------------------------------------------------------------------
+// Operator handle
+// ---------------------------------------------------------------------------
+
+static int pick_jsplit(int64_t n_rows, int64_t n_cols, int rows_per_block) {
+  const int64_t blocks = ceil_div(n_rows, rows_per_block);
+  const int64_t tiles = ceil_div(n_cols, 32);
+  int64_t js = 8;
+  // aim for >= ~2048 blocks of 4 waves; at least 2 column tiles per split
+  while (blocks * js < 2048 && js < 64) js *= 2;
+  while (js > 1 && tiles / js < 2) js /= 2;
+  return (int)js;
+}
+
+static int launch_vec(int mode, const void* colpack, const void* rowpack, int ka,
+                      int64_t n_rows, int64_t n_cols, const float* X, int64_t ldx, int kc,
+                      float* part, int jsplit, bool clamp, bool deriv, float clampv,
+                      hipStream_t st) {
+  const int rows_per_block = 4 * 64;  // ITW=2
+  dim3 grid((unsigned)(ceil_div(n_rows, rows_per_block) * jsplit)), block(256);
+#define LAKER_VEC_LAUNCH(MODE, NS, KC, CL, DV)                                                  \
+  hipLaunchKernelGGL((mv_vec_kernel<MODE, NS, KC, 2, CL, DV>), grid, block, 0, st, colpack, rowpack, \
+                     ka, n_rows, n_cols, X, ldx, kc, part, jsplit, clampv)
+#define LAKER_VEC_KC(MODE, NS, CL, DV)              \
+  do {                                              \
+    if (kc == 1) LAKER_VEC_LAUNCH(MODE, NS, 1, CL, DV); \
+    else if (kc == 2) LAKER_VEC_LAUNCH(MODE, NS, 2, CL, DV); \
+    else LAKER_VEC_LAUNCH(MODE, NS, 4, CL, DV);     \
+  } while (0)
+#define LAKER_VEC_CL(MODE, NS)                                  \
+  do {                                                          \
+    if (deriv) {                                                \
+      if (clamp) LAKER_VEC_KC(MODE, NS, true, true);            \
+      else LAKER_VEC_KC(MODE, NS, false, true);                 \
+    } else {                                                    \
+      if (clamp) LAKER_VEC_KC(MODE, NS, true, false);           \
+      else LAKER_VEC_KC(MODE, NS, false, false);                \
+    }                                                           \
+  } while (0)
+  if (mode == LAKER_PREC_FP32) {
+    const int ns = ka / 2;
+    if (ns <= 4) LAKER_VEC_CL(LAKER_PREC_FP32, 4);
+    else if (ns <= 8) LAKER_VEC_CL(LAKER_PREC_FP32, 8);
+    else if (ns <= 16) LAKER_VEC_CL(LAKER_PREC_FP32, 16);
+    else return fail(LAKER_EINVAL, "embedding_dim > 32 not supported by the FP32 matvec");
+  } else {
+    const int ns = ka / 16;
+    if (ns <= 2) LAKER_VEC_CL(LAKER_PREC_MIXED, 2);
+    else if (ns <= 4) LAKER_VEC_CL(LAKER_PREC_MIXED, 4);
+    else if (ns <= 8) LAKER_VEC_CL(LAKER_PREC_MIXED, 8);
+    else return fail(LAKER_EINVAL, "embedding_dim > 32 not supported by the mixed matvec");
+  }
+#undef LAKER_VEC_CL
+#undef LAKER_VEC_KC
+#undef LAKER_VEC_LAUNCH
+  LAKER_HIP_CHECK(hipGetLastError());
+  return LAKER_OK;
+}
+
+static int launch_multi(int mode, int gx, const void* colpack, const void* rowpack, int ka,
+                        int64_t n_rows, int64_t n_cols, const float* X, int64_t ldx, int kcols,
+                        int c_base, int ncc, float* part, int64_t ldp, int jsplit, bool clamp,
+                        bool deriv, float clampv, hipStream_t st) {
+  const int rows_per_block = 4 * 64;  // ITW=2
+  dim3 grid((unsigned)(ceil_div(n_rows, rows_per_block) * jsplit)), block(256);
+#define LAKER_MULTI_LAUNCH(MODE, NS, GX, NCC, CL, DV)                                             \
+  hipLaunchKernelGGL((mv_multi_kernel<MODE, NS, GX, NCC, 2, CL, DV>), grid, block, 0, st, colpack, \
+                     rowpack, ka, n_rows, n_cols, X, ldx, kcols, c_base, part, ldp, jsplit, clampv)
+#define LAKER_MULTI_NCC(MODE, NS, GX, CL, DV)                   \
+  do {                                                          \
+    if (ncc == 1) LAKER_MULTI_LAUNCH(MODE, NS, GX, 1, CL, DV);  \
+    else LAKER_MULTI_LAUNCH(MODE, NS, GX, 2, CL, DV);           \
+  } while (0)
+#define LAKER_MULTI_CL(MODE, NS, GX)                                        \
+  do {                                                                      \
+    if (deriv) {                                                            \
+      if (clamp) LAKER_MULTI_NCC(MODE, NS, GX, true, true);                 \
+      else LAKER_MULTI_NCC(MODE, NS, GX, false, true);                      \
+    } else {                                                                \
+      if (clamp) LAKER_MULTI_NCC(MODE, NS, GX, true, false);                \
+      else LAKER_MULTI_NCC(MODE, NS, GX, false, false);                     \
+    }                                                                       \
+  } while (0)
+#define LAKER_MULTI_GX(MODE, NS)                                                  \
+  do {                                                                            \
+    if (gx == LAKER_PREC_FP32) LAKER_MULTI_CL(MODE, NS, LAKER_PREC_FP32);         \
+    else if (gx == LAKER_PREC_BF16) LAKER_MULTI_CL(MODE, NS, LAKER_PREC_BF16);    \
+    else LAKER_MULTI_CL(MODE, NS, LAKER_PREC_MIXED);                              \
+  } while (0)
+  if (mode == LAKER_PREC_FP32) {
+    const int ns = ka / 2;
+    if (ns <= 4) LAKER_MULTI_GX(LAKER_PREC_FP32, 4);
+    else if (ns <= 8) LAKER_MULTI_GX(LAKER_PREC_FP32, 8);
+    else if (ns <= 16) LAKER_MULTI_GX(LAKER_PREC_FP32, 16);
+    else return fail(LAKER_EINVAL, "embedding_dim > 32 not supported");
+  } else {
+    const int ns = ka / 16;
+    if (ns <= 2) LAKER_MULTI_GX(LAKER_PREC_MIXED, 2);
+    else if (ns <= 4) LAKER_MULTI_GX(LAKER_PREC_MIXED, 4);
+    else if (ns <= 8) LAKER_MULTI_GX(LAKER_PREC_MIXED, 8);
+    else return fail(LAKER_EINVAL, "embedding_dim > 32 not supported");
+  }
+#undef LAKER_MULTI_GX
+#undef LAKER_MULTI_CL
+#undef LAKER_MULTI_NCC
+#undef LAKER_MULTI_LAUNCH
+  LAKER_HIP_CHECK(hipGetLastError());
+  return LAKER_OK;
+}
+
+template <class XT>
+static int launch_f64(const double* Ecol, const double* Erow, int d, int64_t n_rows, int64_t n_cols,
+                      double scale, bool clamp, double clampv, const XT* X, int64_t ldx, int kcols,
+                      double* part, int64_t ldp, int jsplit, hipStream_t st) {
+  dim3 grid((unsigned)(ceil_div(n_rows, 256) * jsplit)), block(256);
+  for (int c0 = 0; c0 < kcols; c0 += 4) {
+    const int rem = kcols - c0;
+#define LAKER_F64_LAUNCH(KC, DM, CL)                                                          \
+  hipLaunchKernelGGL((mv_f64_kernel<KC, DM, CL, XT>), grid, block, 0, st, Ecol, Erow, d, n_rows, \
+                     n_cols, scale, clampv, X, ldx, kcols, c0, part, ldp, jsplit)
+#define LAKER_F64_DM(KC, CL)                                  \
+  do {                                                        \
+    if (d <= 8) LAKER_F64_LAUNCH(KC, 8, CL);                  \
+    else if (d <= 16) LAKER_F64_LAUNCH(KC, 16, CL);           \
+    else LAKER_F64_LAUNCH(KC, 32, CL);                        \
+  } while (0)
+#define LAKER_F64_KC(CL)                                      \
+  do {                                                        \
+    if (rem == 1) LAKER_F64_DM(1, CL);                        \
+    else if (rem == 2) LAKER_F64_DM(2, CL);                   \
+    else LAKER_F64_DM(4, CL);                                 \
+  } while (0)
+    if (clamp) LAKER_F64_KC(true);
+    else LAKER_F64_KC(false);
+#undef LAKER_F64_KC
+#undef LAKER_F64_DM
+#undef LAKER_F64_LAUNCH
+    LAKER_HIP_CHECK(hipGetLastError());
+  }
+  return LAKER_OK;
+}
+
+// Core dispatcher:  Y (n_rows x k) = [lam*Xr] + exp(min(s*Erow.Ecol^T, c)) X
+// Works for the square operator (rows = cols = E) and the rectangular
+// cross products.  fp32 X/Y for FP32/MIXED/BF16 precisions.
+int fused_product_f32(Operator& op, int precision, const Side& rows, const Side& cols,
+                      const float* X, int k, int64_t ldx, float* Y, int64_t ldy, double lam,
+                      const float* Xr, int64_t ldxr, bool clamp, bool deriv, int accumulate,
+                      hipStream_t st) {
+  LAKER_CHECK_ARG(k >= 1, "k must be >= 1");
+  const float clampv = (float)(op.clamp_limit * kLog2eD);
+  const bool mixed = precision == LAKER_PREC_MIXED || precision == LAKER_PREC_BF16;
+  const void* colpack = mixed ? cols.pack16 : cols.pack32;
+  const void* rowpack = mixed ? rows.pack16 : rows.pack32;
+  const int ka = mixed ? op.ka16 : op.ka32;
+  const int gmode = mixed ? LAKER_PREC_MIXED : LAKER_PREC_FP32;
+  const int64_t n_rows = rows.n, n_cols = cols.n;
+  const int jsplit = pick_jsplit(n_rows, n_cols, 256);
+  if (k <= 4) {
+    LAKER_TRY(op.ensure_part((size_t)jsplit * n_rows * 4 * sizeof(float)));
+    float* part = op.part.as<float>();
+    LAKER_TRY(launch_vec(gmode, colpack, rowpack, ka, n_rows, n_cols, X, ldx, k, part, jsplit,
+                         clamp, deriv, clampv, st));
+    const int kcp = k == 1 ? 1 : (k == 2 ? 2 : 4);
+    int64_t tot = n_rows * k;
+    hipLaunchKernelGGL((mv_reduce_kernel<float, float, float, false>),
+                       dim3((unsigned)ceil_div(tot, 256)), dim3(256), 0, st, part, n_rows, k,
+                       (int64_t)kcp, jsplit, lam, Xr, ldxr, (const float*)nullptr, Y, ldy,
+                       accumulate);
+    LAKER_HIP_CHECK(hipGetLastError());
+    return LAKER_OK;
+  }
+  // multi-RHS on MFMA.  GX precision: FP32 path -> FP32 MFMA; MIXED -> hi/lo split; BF16 -> bf16.
+  const int gx = precision == LAKER_PREC_FP32 ? LAKER_PREC_FP32
+                 : precision == LAKER_PREC_BF16 ? LAKER_PREC_BF16
+                                                : LAKER_PREC_MIXED;
+  // columns per launch: up to 64 (NCC=2) per pass; passes recompute the Gram tile.
+  const int64_t kp = round_up(k, 32);
+  LAKER_TRY(op.ensure_part((size_t)jsplit * n_rows * kp * sizeof(float)));
+  float* part = op.part.as<float>();
+  for (int c0 = 0; c0 < k; c0 += 64) {
+    const int ncc = (k - c0) > 32 ? 2 : 1;
+    LAKER_TRY(launch_multi(gmode, gx, colpack, rowpack, ka, n_rows, n_cols, X, ldx, k, c0, ncc,
+                           part, kp, jsplit, clamp, deriv, clampv, st));
+  }
+  int64_t tot = n_rows * k;
+  hipLaunchKernelGGL((mv_reduce_kernel<float, float, float, false>),
+                     dim3((unsigned)ceil_div(tot, 256)), dim3(256), 0, st, part, n_rows, k, kp,
+                     jsplit, lam, Xr, ldxr, (const float*)nullptr, Y, ldy, accumulate);
+  LAKER_HIP_CHECK(hipGetLastError());
+  return LAKER_OK;
+}
+
+// FP64 product: Y = [lam*Xr] + exp(min(s*g, c)) X  (or residual Y = B - ...).
+template <class XT>
+int fused_product_f64(Operator& op, const Side& rows, const Side& cols, const XT* X, int k,
+                      int64_t ldx, double* Y, int64_t ldy, double lam, const XT* Xr, int64_t ldxr,
+                      bool clamp, const double* B, hipStream_t st) {
+  LAKER_CHECK_ARG(k >= 1, "k must be >= 1");
+  const int64_t n_rows = rows.n, n_cols = cols.n;
+  int jsplit = 8;
+  while (ceil_div(n_rows, 256) * jsplit < 1024 && jsplit < 64) jsplit *= 2;
+  while (jsplit > 1 && ceil_div(n_cols, 64) / jsplit < 2) jsplit /= 2;
+  const int64_t ldp = k;
+  LAKER_TRY(op.ensure_part((size_t)jsplit * n_rows * k * sizeof(double)));
+  double* part = op.part.as<double>();
+  LAKER_TRY(launch_f64<XT>(cols.e64, rows.e64, op.d, n_rows, n_cols, op.scale, clamp,
+                           op.clamp_limit, X, ldx, k, part, ldp, jsplit, st));
+  int64_t tot = n_rows * k;
+  if (B) {
+    hipLaunchKernelGGL((mv_reduce_kernel<double, XT, double, true>),
+                       dim3((unsigned)ceil_div(tot, 256)), dim3(256), 0, st, part, n_rows, k, ldp,
+                       jsplit, lam, Xr, ldxr, B, Y, ldy, 0);
+  } else {
+    hipLaunchKernelGGL((mv_reduce_kernel<double, XT, double, false>),
+                       dim3((unsigned)ceil_div(tot, 256)), dim3(256), 0, st, part, n_rows, k, ldp,
+                       jsplit, lam, Xr, ldxr, (const double*)nullptr, Y, ldy, 0);
+  }
+  LAKER_HIP_CHECK(hipGetLastError());
+  return LAKER_OK;
+}
+template int fused_product_f64<float>(Operator&, const Side&, const Side&, const float*, int,
+                                      int64_t, double*, int64_t, double, const float*, int64_t,
+                                      bool, const double*, hipStream_t);
+template int fused_product_f64<double>(Operator&, const Side&, const Side&, const double*, int,
+                                       int64_t, double*, int64_t, double, const double*, int64_t,
+                                       bool, const double*, hipStream_t);
+
+// Build the packed operand arrays of one embedding set.
+template <class TE>
+static int pack_store_t(SideStore& s, const TE* E, int64_t n, int d, int ka32, int ka16,
+                        double cscale, bool need_col, bool need_row, hipStream_t st) {
+  s.n = n;
+  s.npad = round_up(n, 64);
+  const int64_t n32 = s.npad * ka32, n16 = s.npad * ka16;
+  if (need_col) {
+    LAKER_TRY(s.col32.ensure(n32 * 4));
+    LAKER_TRY(s.col16.ensure(n16 * 2));
+  }
+  if (need_row) {
+    LAKER_TRY(s.row32.ensure(n32 * 4));
+    LAKER_TRY(s.row16.ensure(n16 * 2));
+  }
+  LAKER_TRY(s.e64.ensure(n * d * 8));
+  LAKER_TRY(s.user.ensure(n * d * sizeof(TE)));
+  LAKER_HIP_CHECK(hipMemcpyAsync(s.user.ptr, E, n * d * sizeof(TE), hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL((pack_f32_kernel<TE>), dim3((unsigned)ceil_div(n32, 256)), dim3(256), 0, st, E,
+                     n, d, s.npad, ka32, (float)cscale, need_col ? s.col32.as<float>() : nullptr,
+                     need_row ? s.row32.as<float>() : nullptr);
+  hipLaunchKernelGGL((pack_bf16x4_kernel<TE>), dim3((unsigned)ceil_div(n16, 256)), dim3(256), 0, st,
+                     E, n, d, s.npad, ka16, (float)cscale,
+                     need_col ? s.col16.as<uint16_t>() : nullptr,
+                     need_row ? s.row16.as<uint16_t>() : nullptr);
+  hipLaunchKernelGGL((pack_f64_kernel<TE>), dim3((unsigned)ceil_div(n * d, 256)), dim3(256), 0, st,
+                     E, n, d, s.e64.as<double>());
+  LAKER_HIP_CHECK(hipGetLastError());
+  return LAKER_OK;
+}
+
+int pack_store(SideStore& s, const void* E, int dtype, int64_t n, int d, int ka32, int ka16,
+               double scale, bool need_col, bool need_row, hipStream_t st) {
+  const double cscale = scale * kLog2eD;
+  if (dtype == LAKER_F32)
+    return pack_store_t<float>(s, (const float*)E, n, d, ka32, ka16, cscale, need_col, need_row, st);
+  return pack_store_t<double>(s, (const double*)E, n, d, ka32, ka16, cscale, need_col, need_row, st);
+}
+
+}  // namespace laker
+
+using namespace laker;
+
+extern "C" {
+
+int laker_operator_create(const void* E, int64_t n, int32_t d, int32_t dtype, double lambda_reg,
+                          double scale, int32_t clamp_policy, hipStream_t stream,
+                          laker_operator_t* out) {
+  LAKER_CHECK_ARG(out != nullptr, "out handle is NULL");
+  LAKER_CHECK_ARG(E != nullptr || n == 0, "embeddings pointer is NULL");
+  LAKER_CHECK_ARG(n >= 1, "embeddings must have at least one row");
+  LAKER_CHECK_ARG(d >= 1 && d <= 32, "embedding_dim must be in [1, 32]");
+  LAKER_CHECK_ARG(dtype == LAKER_F32 || dtype == LAKER_F64, "dtype must be LAKER_F32 or LAKER_F64");
+  LAKER_CHECK_ARG(clamp_policy >= 0 && clamp_policy <= 2, "bad clamp policy");
+  Operator* op = new Operator();
+  op->n = n;
+  op->d = d;
+  op->dtype = dtype;
+  op->lambda_reg = lambda_reg;
+  op->scale = scale;
+  op->clamp_policy = clamp_policy;
+  op->clamp_limit = dtype == LAKER_F32 ? 80.0 : 700.0;
+  op->ka32 = (int)round_up(d, 2);
+  op->ka16 = (int)round_up(4 * d, 16);
+  int rc = pack_store(op->self, E, dtype, n, d, op->ka32, op->ka16, scale, true, true, stream);
+  if (rc != LAKER_OK) {
+    delete op;
+    return rc;
+  }
+  // skip-clamp predicate: max ||e||^2 (times scale) < limit
+  DevBuf mx;
+  if ((rc = mx.ensure(8)) != LAKER_OK) {
+    delete op;
+    return rc;
+  }
+  hipError_t e = hipMemsetAsync(mx.ptr, 0, 8, stream);
+  if (e == hipSuccess) {
+    if (dtype == LAKER_F32)
+      hipLaunchKernelGGL((max_sqnorm_kernel<float>), dim3((unsigned)ceil_div(n, 256)), dim3(256), 0,
+                         stream, (const float*)E, n, d, mx.as<unsigned long long>());
+    else
+      hipLaunchKernelGGL((max_sqnorm_kernel<double>), dim3((unsigned)ceil_div(n, 256)), dim3(256), 0,
+                         stream, (const double*)E, n, d, mx.as<unsigned long long>());
+    e = hipGetLastError();
+  }
+  uint64_t bits = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&bits, mx.ptr, 8, hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) {
+    delete op;
+    return fail(LAKER_EHIP, std::string("HIP error in operator create: ") + hipGetErrorString(e));
+  }
+  op->max_sq_norm = __builtin_bit_cast(double, bits);
+  op->skip_clamp = (op->max_sq_norm * scale) < op->clamp_limit;
+  *out = reinterpret_cast<laker_operator_t>(op);
+  return LAKER_OK;
+}
+
+int laker_operator_destroy(laker_operator_t h) {
+  delete reinterpret_cast<Operator*>(h);
+  return LAKER_OK;
+}
+
+int laker_operator_info(laker_operator_t h, int32_t* skip_clamp, double* max_sq_norm) {
+  LAKER_CHECK_ARG(h != nullptr, "operator handle is NULL");
+  Operator* op = reinterpret_cast<Operator*>(h);
+  if (skip_clamp) *skip_clamp = op->skip_clamp ? 1 : 0;
+  if (max_sq_norm) *max_sq_norm = op->max_sq_norm;
+  return LAKER_OK;
+}
+
+int laker_operator_set_lambda(laker_operator_t h, double lambda_reg) {
+  LAKER_CHECK_ARG(h != nullptr, "operator handle is NULL");
+  reinterpret_cast<Operator*>(h)->lambda_reg = lambda_reg;
+  return LAKER_OK;
+}
+
+int laker_matvec(laker_operator_t h, const void* X, int32_t k, int64_t ldx, void* Y, int64_t ldy,
+                 int32_t precision, hipStream_t stream) {
+  LAKER_CHECK_ARG(h != nullptr, "operator handle is NULL");
+  LAKER_CHECK_ARG(X != nullptr && Y != nullptr, "X/Y pointer is NULL");
+  LAKER_CHECK_ARG(k >= 1 && ldx >= k && ldy >= k, "bad k / leading dimension");
+  Operator* op = reinterpret_cast<Operator*>(h);
+  const bool clamp = op->clamp_for(LAKER_CLAMP_AUTO);
+  Side rows = op->rows(), cols = op->cols();
+  if (op->dtype == LAKER_F64 || precision == LAKER_PREC_FP64) {
+    LAKER_CHECK_ARG(op->dtype == LAKER_F64, "LAKER_PREC_FP64 matvec needs an FP64 operator; use laker_residual_fp64");
+    return fused_product_f64<double>(*op, rows, cols, (const double*)X, k, ldx, (double*)Y, ldy,
+                                     op->lambda_reg, (const double*)X, ldx, clamp, nullptr, stream);
+  }
+  LAKER_CHECK_ARG(precision == LAKER_PREC_FP32 || precision == LAKER_PREC_MIXED ||
+                      precision == LAKER_PREC_BF16,
+                  "bad precision");
+  return fused_product_f32(*op, precision, rows, cols, (const float*)X, k, ldx, (float*)Y, ldy,
+                           op->lambda_reg, (const float*)X, ldx, clamp, false, 0, stream);
+}
+
+int laker_residual_fp64(laker_operator_t h, const double* B, const double* X, int32_t k, double* R,
+                        hipStream_t stream) {
+  LAKER_CHECK_ARG(h != nullptr, "operator handle is NULL");
+  LAKER_CHECK_ARG(B && X && R && k >= 1, "bad residual arguments");
+  Operator* op = reinterpret_cast<Operator*>(h);
+  const bool clamp = op->clamp_for(LAKER_CLAMP_AUTO);
+  Side rows = op->rows(), cols = op->cols();
+  return fused_product_f64<double>(*op, rows, cols, X, k, k, R, k, op->lambda_reg, X, k, clamp, B,
+                                   stream);
+}
+
+int laker_cross_matvec(laker_operator_t h, const void* Eq, int64_t m, const void* X, int32_t k,
+                       int64_t ldx, void* Y, int64_t ldy, int32_t clamp_policy, int32_t precision,
+                       hipStream_t stream) {
+  LAKER_CHECK_ARG(h != nullptr, "operator handle is NULL");
+  LAKER_CHECK_ARG(Eq && X && Y && m >= 1 && k >= 1, "bad cross_matvec arguments");
+  Operator* op = reinterpret_cast<Operator*>(h);
+  const bool clamp = op->clamp_for(clamp_policy);
+  std::lock_guard<std::mutex> g(op->query_mu);
+  LAKER_TRY(pack_store(op->query, Eq, op->dtype, m, op->d, op->ka32, op->ka16, op->scale, false,
+                       true, stream));
+  Side qrows = Operator::view(op->query, true);
+  Side cols = op->cols();
+  if (op->dtype == LAKER_F64 || precision == LAKER_PREC_FP64) {
+    LAKER_CHECK_ARG(op->dtype == LAKER_F64, "FP64 cross product needs an FP64 operator");
+    return fused_product_f64<double>(*op, qrows, cols, (const double*)X, k, ldx, (double*)Y,
+                                     ldy, 0.0, nullptr, 0, clamp, nullptr, stream);
+  }
+  return fused_product_f32(*op, precision, qrows, cols, (const float*)X, k, ldx, (float*)Y, ldy,
+                           0.0, nullptr, 0, clamp, false, 0, stream);
+}
+
+int laker_cross_matvec_t(laker_operator_t h, const void* Eq, int64_t m, const void* Xq, int32_t k,
+                         int64_t ldx, void* Y, int64_t ldy, int32_t clamp_policy,
+                         hipStream_t stream) {
+  LAKER_CHECK_ARG(h != nullptr, "operator handle is NULL");
+  LAKER_CHECK_ARG(Eq && Xq && Y && m >= 1 && k >= 1, "bad cross_matvec_t arguments");
+  Operator* op = reinterpret_cast<Operator*>(h);
+  const bool clamp = op->clamp_for(clamp_policy);
+  std::lock_guard<std::mutex> g(op->query_mu);
+  LAKER_TRY(pack_store(op->query, Eq, op->dtype, m, op->d, op->ka32, op->ka16, op->scale, true,
+                       false, stream));
+  Side qcols = Operator::view(op->query, false);
+  Side rows = op->rows();
+  if (op->dtype == LAKER_F64)
+    return fused_product_f64<double>(*op, rows, qcols, (const double*)Xq, k, ldx, (double*)Y,
+                                     ldy, 0.0, nullptr, 0, clamp, nullptr, stream);
+  return fused_product_f32(*op, LAKER_PREC_FP32, rows, qcols, (const float*)Xq, k, ldx,
+                           (float*)Y, ldy, 0.0, nullptr, 0, clamp, false, 0, stream);
+}
+
+int laker_kernel_eval(const void* X, int64_t m, const void* Y, int64_t p, int32_t d, int32_t dtype,
+                      double scale, int32_t clamp, void* K, int64_t ldk, hipStream_t stream) {
+  LAKER_CHECK_ARG(X && Y && K && m >= 1 && p >= 1 && d >= 1 && ldk >= p, "bad kernel_eval arguments");
+  dim3 grid = eval_grid(m * p);
+  if (dtype == LAKER_F32)
+    hipLaunchKernelGGL((kernel_eval_kernel<float>), grid, dim3(256), 0, stream, (const float*)X, m,
+                       (const float*)Y, p, d, scale, clamp, 80.0, 0.0, (float*)K, ldk);
+  else
+    hipLaunchKernelGGL((kernel_eval_kernel<double>), grid, dim3(256), 0, stream, (const double*)X, m,
+                       (const double*)Y, p, d, scale, clamp, 700.0, 0.0, (double*)K, ldk);
+  LAKER_HIP_CHECK(hipGetLastError());
+  return LAKER_OK;
+}
+
+int laker_diagonal(laker_operator_t h, void* out, hipStream_t stream) {
+  LAKER_CHECK_ARG(h != nullptr && out != nullptr, "bad diagonal arguments");
+  Operator* op = reinterpret_cast<Operator*>(h);
+  if (op->dtype == LAKER_F32)
+    hipLaunchKernelGGL((diag_kernel<float>), dim3((unsigned)ceil_div(op->n, 256)), dim3(256), 0,
+                       stream, op->e_user32(), op->n, op->d, op->scale, op->lambda_reg, (float*)out);
+  else
+    hipLaunchKernelGGL((diag_kernel<double>), dim3((unsigned)ceil_div(op->n, 256)), dim3(256), 0,
+                       stream, op->self.e64.as<double>(), op->n, op->d, op->scale, op->lambda_reg,
+                       (double*)out);
+  LAKER_HIP_CHECK(hipGetLastError());
+  return LAKER_OK;
+}
+
+int laker_to_dense(laker_operator_t h, void* out, hipStream_t stream) {
+  LAKER_CHECK_ARG(h != nullptr && out != nullptr, "bad to_dense arguments");
+  Operator* op = reinterpret_cast<Operator*>(h);
+  dim3 grid = eval_grid(op->n * op->n);
+  if (op->dtype == LAKER_F32)
+    hipLaunchKernelGGL((kernel_eval_kernel<float>), grid, dim3(256), 0, stream, op->e_user32(),
+                       op->n, op->e_user32(), op->n, op->d, op->scale, 0, 80.0, op->lambda_reg,
+                       (float*)out, op->n);
+  else
+    hipLaunchKernelGGL((kernel_eval_kernel<double>), grid, dim3(256), 0, stream,
+                       op->self.e64.as<double>(), op->n, op->self.e64.as<double>(), op->n, op->d,
+                       op->scale, 0, 700.0, op->lambda_reg, (double*)out, op->n);
+  LAKER_HIP_CHECK(hipGetLastError());
+  return LAKER_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,497 @@
+// Device-resident preconditioned conjugate gradient and the mixed-precision
+// (BF16-Gram inner PCG + FP64 iterative refinement) solve.
+//
+// Restates laker/solvers.py:130-394: zero-RHS early return, warm start with
+// r = b - A x0, breakdown test p^T A p <= -eps ||p|| ||Ap|| (RuntimeError in
+// the reference, LAKER_EBREAKDOWN here), alpha/beta recurrences (per column for
+// k > 1), optional residual replacement every restart_freq iterations, and the
+// relative-residual stop ||r||_F / ||b||_F <= tol with iterations = it + 1.
+// Scalars live on device in FP64; vectors stay resident; the host only polls a
+// status word every `check_every` iterations, and every kernel after the
+// stopping point is a no-op.
+#include <cmath>
+#include <vector>
+
+#include "dense64.h"
+#include "laker_common.h"
+#include "laker_internal.h"
+
+namespace laker {
+
+int precond_apply(struct Precond& pc, const void* X, int k, int64_t ldx, void* Z, int64_t ldz,
+                  hipStream_t st);
+
+enum { PCG_RUNNING = 0, PCG_CONVERGED = 1, PCG_BREAKDOWN = 2, PCG_MAXITER = 3 };
+
+struct PcgState {
+  int32_t done;
+  int32_t iterations;
+  double res_norm;
+  double rhs_norm;
+  double tol;
+  double eps;
+};
+
+constexpr int DOT_TC = 64;   // columns per thread group (k > 1)
+constexpr int DOT_NB = 512;  // max partial blocks
+
+// part[q][blk][c] = sum over the block's rows of A_q[i][c] * B_q[i][c]
+template <class T, int NQ>
+__global__ __launch_bounds__(256) void coldots_kernel(const T* __restrict__ A0, const T* __restrict__ B0,
+                                                      const T* __restrict__ A1, const T* __restrict__ B1,
+                                                      const T* __restrict__ A2, const T* __restrict__ B2,
+                                                      int64_t n, int k, int64_t rpb,
+                                                      double* __restrict__ part, int nblk,
+                                                      const PcgState* __restrict__ st) {
+  if (st && st->done) return;
+  __shared__ double sm[NQ][256];
+  const int64_t r0 = (int64_t)blockIdx.x * rpb;
+  const int64_t r1 = (r0 + rpb < n) ? r0 + rpb : n;
+  if (k == 1) {
+    double acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) {
+      acc[0] += (double)A0[i] * (double)B0[i];
+      if (NQ > 1) acc[1] += (double)A1[i] * (double)B1[i];
+      if (NQ > 2) acc[2] += (double)A2[i] * (double)B2[i];
+    }
+    __shared__ double red[16];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const double s = block_sum(acc[q], red);
+      if (threadIdx.x == 0) part[(int64_t)q * nblk + blockIdx.x] = s;
+    }
+    return;
+  }
+  const int tc = threadIdx.x % DOT_TC, tr = threadIdx.x / DOT_TC;  // 4 row lanes
+  for (int c0 = 0; c0 < k; c0 += DOT_TC) {
+    const int c = c0 + tc;
+    double acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+    if (c < k) {
+      for (int64_t i = r0 + tr; i < r1; i += 4) {
+        const int64_t o = i * k + c;
+        acc[0] += (double)A0[o] * (double)B0[o];
+        if (NQ > 1) acc[1] += (double)A1[o] * (double)B1[o];
+        if (NQ > 2) acc[2] += (double)A2[o] * (double)B2[o];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) sm[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    if (tr == 0 && c < k) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const double s = sm[q][tc] + sm[q][tc + 64] + sm[q][tc + 128] + sm[q][tc + 192];
+        part[((int64_t)q * nblk + blockIdx.x) * k + c] = s;
+      }
+    }
+  }
+}
+
+// out[q][c] = sum_blk part[q][blk][c]  (fixed order)
+__device__ inline double sum_part(const double* part, int q, int nblk, int k, int c) {
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += part[((int64_t)q * nblk + b) * k + c];
+  return s;
+}
+
+// alpha = rz / pAp with the breakdown test.  One block.
+__global__ void pcg_alpha_kernel(const double* __restrict__ part, int nblk, int k,
+                                 const double* __restrict__ rz, double* __restrict__ alpha,
+                                 PcgState* __restrict__ st) {
+  if (st->done) return;
+  __shared__ int brk;
+  if (threadIdx.x == 0) brk = 0;
+  __syncthreads();
+  for (int c = threadIdx.x; c < k; c += blockDim.x) {
+    const double pap = sum_part(part, 0, nblk, k, c);
+    const double pp = sum_part(part, 1, nblk, k, c);
+    const double aa = sum_part(part, 2, nblk, k, c);
+    if (pap <= -st->eps * sqrt(pp) * sqrt(aa)) atomicOr(&brk, 1);
+    alpha[c] = rz[c] / pap;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && brk) st->done = PCG_BREAKDOWN;
+}
+
+// x += alpha p ; r -= alpha Ap
+template <class T>
+__global__ void pcg_update_xr_kernel(T* __restrict__ x, T* __restrict__ r, const T* __restrict__ p,
+                                     const T* __restrict__ ap, const double* __restrict__ alpha,
+                                     int64_t n, int k, const PcgState* __restrict__ st) {
+  if (st->done) return;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * k) return;
+  const int c = (int)(idx % k);
+  const double a = alpha[c];
+  x[idx] = (T)((double)x[idx] + a * (double)p[idx]);
+  r[idx] = (T)((double)r[idx] - a * (double)ap[idx]);
+}
+
+// residual norm, convergence and max-iteration bookkeeping.  One block.
+__global__ void pcg_resid_kernel(const double* __restrict__ part, int nblk, int k, int it,
+                                 int max_iter, PcgState* __restrict__ st) {
+  if (st->done) return;
+  __shared__ double red[16];
+  double s = 0.0;
+  for (int c = threadIdx.x; c < k; c += blockDim.x) s += sum_part(part, 0, nblk, k, c);
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) {
+    const double res = sqrt(s);
+    st->res_norm = res;
+    if (res / st->rhs_norm <= st->tol) {
+      st->done = PCG_CONVERGED;
+      st->iterations = it + 1;
+    } else if (it + 1 >= max_iter) {
+      st->done = PCG_MAXITER;
+      st->iterations = max_iter;
+    }
+  }
+}
+
+// beta = rz_new / rz_old ; rz_old = rz_new.  One block.
+__global__ void pcg_beta_kernel(const double* __restrict__ part, int nblk, int k, double* __restrict__ rz,
+                                double* __restrict__ beta, const PcgState* __restrict__ st) {
+  if (st->done) return;
+  for (int c = threadIdx.x; c < k; c += blockDim.x) {
+    const double v = sum_part(part, 0, nblk, k, c);
+    beta[c] = v / rz[c];
+    rz[c] = v;
+  }
+}
+
+// p = z + beta p
+template <class T>
+__global__ void pcg_update_p_kernel(T* __restrict__ p, const T* __restrict__ z,
+                                    const double* __restrict__ beta, int64_t n, int k,
+                                    const PcgState* __restrict__ st) {
+  if (st->done) return;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * k) return;
+  const int c = (int)(idx % k);
+  p[idx] = (T)((double)z[idx] + beta[c] * (double)p[idx]);
+}
+
+// set-up scalars: rz = colsum(r z), rhs_norm = ||b||_F.  One block.
+__global__ void pcg_setup_kernel(const double* __restrict__ part, int nblk, int k, double* __restrict__ rz,
+                                 PcgState* __restrict__ st, double tol, double eps) {
+  __shared__ double red[16];
+  double s = 0.0;
+  for (int c = threadIdx.x; c < k; c += blockDim.x) {
+    rz[c] = sum_part(part, 0, nblk, k, c);
+    s += sum_part(part, 1, nblk, k, c);
+  }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) {
+    st->done = PCG_RUNNING;
+    st->iterations = 0;
+    st->rhs_norm = sqrt(s);
+    st->res_norm = INFINITY;
+    st->tol = tol;
+    st->eps = eps;
+  }
+}
+
+template <class T>
+__global__ void axpby_kernel(const T* __restrict__ a, double fa, const T* __restrict__ b, double fb,
+                             T* __restrict__ out, int64_t count, const PcgState* __restrict__ st) {
+  if (st && st->done) return;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= count) return;
+  double v = fa * (double)a[idx];
+  if (b) v += fb * (double)b[idx];
+  out[idx] = (T)v;
+}
+
+// ---------------------------------------------------------------------------
+template <class T>
+struct PcgBuffers {
+  T *x, *r, *z, *p, *ap, *tmp;
+  double *part, *rz, *alpha, *beta;
+  PcgState* st;
+  int nblk;
+  int64_t rpb;
+};
+
+static int pcg_nblk(int64_t n, int64_t* rpb) {
+  int64_t nb = std::min<int64_t>(DOT_NB, ceil_div(n, 256));
+  *rpb = ceil_div(n, nb);
+  return (int)ceil_div(n, *rpb);
+}
+
+template <class T>
+static int coldots(const PcgBuffers<T>& b, int nq, const T* a0, const T* b0, const T* a1, const T* b1,
+                   const T* a2, const T* b2, int64_t n, int k, bool check, hipStream_t st) {
+  const PcgState* s = check ? b.st : nullptr;
+  if (nq == 1)
+    hipLaunchKernelGGL((coldots_kernel<T, 1>), dim3(b.nblk), dim3(256), 0, st, a0, b0, a1, b1, a2, b2, n,
+                       k, b.rpb, b.part, b.nblk, s);
+  else if (nq == 2)
+    hipLaunchKernelGGL((coldots_kernel<T, 2>), dim3(b.nblk), dim3(256), 0, st, a0, b0, a1, b1, a2, b2, n,
+                       k, b.rpb, b.part, b.nblk, s);
+  else
+    hipLaunchKernelGGL((coldots_kernel<T, 3>), dim3(b.nblk), dim3(256), 0, st, a0, b0, a1, b1, a2, b2, n,
+                       k, b.rpb, b.part, b.nblk, s);
+  LAKER_HIP_CHECK(hipGetLastError());
+  return LAKER_OK;
+}
+
+// A p for the operator in its dtype and the requested precision.
+template <class T>
+static int op_apply(Operator& op, int precision, const T* X, int k, T* Y, const int* skip,
+                    hipStream_t st) {
+  (void)skip;
+  const bool clamp = op.clamp_for(LAKER_CLAMP_AUTO);
+  if constexpr (sizeof(T) == 8) {
+    return fused_product_f64<double>(op, op.rows(), op.cols(), (const double*)X, k, k, (double*)Y, k,
+                                     op.lambda_reg, (const double*)X, k, clamp, nullptr, st);
+  } else {
+    return fused_product_f32(op, precision, op.rows(), op.cols(), (const float*)X, k, k, (float*)Y, k,
+                             op.lambda_reg, (const float*)X, k, clamp, false, 0, st);
+  }
+}
+
+template <class T>
+__global__ void copy_kernel(const T* __restrict__ a, T* __restrict__ out, int64_t count) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < count) out[idx] = a[idx];
+}
+
+template <class T>
+static int pcg_run(Operator& op, Precond* pc, const T* B, int k, T* X, const T* X0, double tol,
+                   int max_iter, double eps, int restart_freq, int precision, int check_every,
+                   hipStream_t st, laker_pcg_result* res) {
+  const int64_t n = op.n;
+  const int64_t nk = n * k;
+  int64_t rpb;
+  const int nblk = pcg_nblk(n, &rpb);
+  const size_t vec_bytes = round_up(nk * sizeof(T), 256);
+  const size_t part_bytes = round_up((size_t)3 * nblk * k * 8, 256);
+  const size_t col_bytes = round_up((size_t)k * 8, 256);
+  const size_t need = 5 * vec_bytes + part_bytes + 3 * col_bytes + 256;
+  LAKER_TRY(op.pcg_ws.ensure(need));
+  char* base = op.pcg_ws.as<char>();
+  PcgBuffers<T> b;
+  b.x = X;
+  b.r = (T*)base;
+  b.z = (T*)(base + vec_bytes);
+  b.p = (T*)(base + 2 * vec_bytes);
+  b.ap = (T*)(base + 3 * vec_bytes);
+  b.tmp = (T*)(base + 4 * vec_bytes);
+  b.part = (double*)(base + 5 * vec_bytes);
+  b.rz = (double*)(base + 5 * vec_bytes + part_bytes);
+  b.alpha = (double*)(base + 5 * vec_bytes + part_bytes + col_bytes);
+  b.beta = (double*)(base + 5 * vec_bytes + part_bytes + 2 * col_bytes);
+  b.st = (PcgState*)(base + 5 * vec_bytes + part_bytes + 3 * col_bytes);
+  b.nblk = nblk;
+  b.rpb = rpb;
+  const dim3 eg((unsigned)ceil_div(nk, 256)), eb(256);
+
+  // set-up: x = x0 or 0 ; r = b - A x0 or b ; z = P r ; p = z
+  if (X0) {
+    hipLaunchKernelGGL((copy_kernel<T>), eg, eb, 0, st, X0, b.x, nk);
+    LAKER_TRY(op_apply<T>(op, precision, b.x, k, b.tmp, nullptr, st));
+    hipLaunchKernelGGL((axpby_kernel<T>), eg, eb, 0, st, B, 1.0, (const T*)b.tmp, -1.0, b.r, nk,
+                       (const PcgState*)nullptr);
+  } else {
+    LAKER_HIP_CHECK(hipMemsetAsync(b.x, 0, nk * sizeof(T), st));
+    hipLaunchKernelGGL((copy_kernel<T>), eg, eb, 0, st, B, b.r, nk);
+  }
+  LAKER_HIP_CHECK(hipGetLastError());
+  if (pc) LAKER_TRY(precond_apply(*pc, b.r, k, k, b.z, k, st));
+  else hipLaunchKernelGGL((copy_kernel<T>), eg, eb, 0, st, (const T*)b.r, b.z, nk);
+  hipLaunchKernelGGL((copy_kernel<T>), eg, eb, 0, st, (const T*)b.z, b.p, nk);
+  LAKER_TRY(coldots<T>(b, 2, b.r, b.z, B, B, nullptr, nullptr, n, k, false, st));
+  hipLaunchKernelGGL(pcg_setup_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.st, tol, eps);
+  LAKER_HIP_CHECK(hipGetLastError());
+  PcgState hs;
+  LAKER_HIP_CHECK(hipMemcpyAsync(&hs, b.st, sizeof(hs), hipMemcpyDeviceToHost, st));
+  LAKER_HIP_CHECK(hipStreamSynchronize(st));
+  res->rhs_norm = hs.rhs_norm;
+  res->zero_rhs = 0;
+  if (hs.rhs_norm == 0.0) {
+    res->iterations = 0;
+    res->status = LAKER_OK;
+    res->converged = 1;
+    res->zero_rhs = 1;
+    res->residual_norm = INFINITY;
+    return LAKER_OK;
+  }
+  if (max_iter <= 0) {
+    res->iterations = 0;
+    res->status = LAKER_OK;
+    res->converged = 0;
+    res->residual_norm = INFINITY;
+    return LAKER_OK;
+  }
+  if (check_every < 1) check_every = 1;
+  for (int it = 0; it < max_iter; ++it) {
+    LAKER_TRY(op_apply<T>(op, precision, b.p, k, b.ap, &b.st->done, st));
+    LAKER_TRY(coldots<T>(b, 3, b.p, b.ap, b.p, b.p, b.ap, b.ap, n, k, true, st));
+    hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.alpha, b.st);
+    hipLaunchKernelGGL((pcg_update_xr_kernel<T>), eg, eb, 0, st, b.x, b.r, b.p, b.ap, b.alpha, n, k, b.st);
+    LAKER_HIP_CHECK(hipGetLastError());
+    if (restart_freq > 0 && (it + 1) % restart_freq == 0) {
+      // residual replacement r = b - A x
+      LAKER_TRY(op_apply<T>(op, precision, b.x, k, b.tmp, &b.st->done, st));
+      hipLaunchKernelGGL((axpby_kernel<T>), eg, eb, 0, st, B, 1.0, (const T*)b.tmp, -1.0, b.r, nk,
+                         (const PcgState*)b.st);
+    }
+    LAKER_TRY(coldots<T>(b, 1, b.r, b.r, nullptr, nullptr, nullptr, nullptr, n, k, true, st));
+    hipLaunchKernelGGL(pcg_resid_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, it, max_iter, b.st);
+    if (pc) LAKER_TRY(precond_apply(*pc, b.r, k, k, b.z, k, st));
+    else hipLaunchKernelGGL((copy_kernel<T>), eg, eb, 0, st, (const T*)b.r, b.z, nk);
+    LAKER_TRY(coldots<T>(b, 1, b.r, b.z, nullptr, nullptr, nullptr, nullptr, n, k, true, st));
+    hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.beta, b.st);
+    hipLaunchKernelGGL((pcg_update_p_kernel<T>), eg, eb, 0, st, b.p, b.z, b.beta, n, k, b.st);
+    LAKER_HIP_CHECK(hipGetLastError());
+    if ((it + 1) % check_every == 0 || it + 1 == max_iter) {
+      LAKER_HIP_CHECK(hipMemcpyAsync(&hs, b.st, sizeof(hs), hipMemcpyDeviceToHost, st));
+      LAKER_HIP_CHECK(hipStreamSynchronize(st));
+      if (hs.done) break;
+    }
+  }
+  LAKER_HIP_CHECK(hipMemcpyAsync(&hs, b.st, sizeof(hs), hipMemcpyDeviceToHost, st));
+  LAKER_HIP_CHECK(hipStreamSynchronize(st));
+  res->iterations = hs.done ? hs.iterations : max_iter;
+  res->residual_norm = hs.res_norm;
+  res->converged = hs.done == PCG_CONVERGED;
+  res->status = hs.done == PCG_BREAKDOWN ? LAKER_EBREAKDOWN : LAKER_OK;
+  if (hs.done == PCG_BREAKDOWN) {
+    res->iterations = hs.iterations;
+    return fail(LAKER_EBREAKDOWN,
+                "PCG breakdown: non-positive curvature detected (p^T A p <= 0). The operator may be "
+                "indefinite or the preconditioner may be unsuitable.");
+  }
+  return LAKER_OK;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 helpers of the refinement loop
+// ---------------------------------------------------------------------------
+__global__ void frob2_part_kernel(const double* __restrict__ a, int64_t count, double* __restrict__ part) {
+  __shared__ double red[16];
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x)
+    s += a[i] * a[i];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+static int frob_norm(const double* a, int64_t count, DevBuf& tmp, double* out, hipStream_t st) {
+  const int nb = (int)std::min<int64_t>(256, ceil_div(count, 256));
+  LAKER_TRY(tmp.ensure((size_t)nb * 8));
+  hipLaunchKernelGGL(frob2_part_kernel, dim3(nb), dim3(256), 0, st, a, count, tmp.as<double>());
+  LAKER_HIP_CHECK(hipGetLastError());
+  std::vector<double> h(nb);
+  LAKER_HIP_CHECK(hipMemcpyAsync(h.data(), tmp.ptr, nb * 8, hipMemcpyDeviceToHost, st));
+  LAKER_HIP_CHECK(hipStreamSynchronize(st));
+  double s = 0.0;
+  for (double v : h) s += v;
+  *out = std::sqrt(s);
+  return LAKER_OK;
+}
+
+__global__ void scale_to_f32_kernel(const double* __restrict__ a, float* __restrict__ out, int64_t count,
+                                    double f) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) out[i] = (float)(a[i] * f);
+}
+__global__ void add_scaled_f32_kernel(double* __restrict__ x, const float* __restrict__ d, int64_t count,
+                                      double f) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) x[i] += f * (double)d[i];
+}
+
+}  // namespace laker
+
+using namespace laker;
+
+extern "C" {
+
+int laker_pcg_solve(laker_operator_t oh, laker_precond_t ph, const void* B, int32_t k, void* X,
+                    const void* X0, double tol, int32_t max_iter, double breakdown_eps,
+                    int32_t restart_freq, int32_t precision, int32_t check_every, hipStream_t stream,
+                    laker_pcg_result* result) {
+  LAKER_CHECK_ARG(oh != nullptr, "operator handle is NULL");
+  LAKER_CHECK_ARG(B && X && result && k >= 1, "bad pcg arguments");
+  Operator& op = *reinterpret_cast<Operator*>(oh);
+  Precond* pc = reinterpret_cast<Precond*>(ph);
+  if (op.dtype == LAKER_F64) {
+    const double eps = breakdown_eps >= 0 ? breakdown_eps : std::sqrt(2.220446049250313e-16);
+    return pcg_run<double>(op, pc, (const double*)B, k, (double*)X, (const double*)X0, tol, max_iter, eps,
+                           restart_freq, LAKER_PREC_FP64, check_every, stream, result);
+  }
+  const double eps = breakdown_eps >= 0 ? breakdown_eps : std::sqrt(1.1920928955078125e-07);
+  LAKER_CHECK_ARG(precision == LAKER_PREC_FP32 || precision == LAKER_PREC_MIXED, "bad PCG precision");
+  return pcg_run<float>(op, pc, (const float*)B, k, (float*)X, (const float*)X0, tol, max_iter, eps,
+                        restart_freq, precision, check_every, stream, result);
+}
+
+int laker_solve_mixed(laker_operator_t oh, laker_precond_t ph, const double* B, int32_t k, double* X,
+                      double tol, int32_t max_outer, double inner_tol, int32_t inner_max_iter,
+                      int32_t inner_precision, hipStream_t st, laker_mixed_result* result) {
+  LAKER_CHECK_ARG(oh != nullptr, "operator handle is NULL");
+  LAKER_CHECK_ARG(B && X && result && k >= 1, "bad mixed-solve arguments");
+  Operator& op = *reinterpret_cast<Operator*>(oh);
+  LAKER_CHECK_ARG(op.dtype == LAKER_F32, "mixed solve needs an FP32 operator");
+  LAKER_CHECK_ARG(inner_precision == LAKER_PREC_MIXED || inner_precision == LAKER_PREC_FP32,
+                  "inner precision must be LAKER_PREC_MIXED or LAKER_PREC_FP32");
+  Precond* pc = reinterpret_cast<Precond*>(ph);
+  const int64_t nk = op.n * k;
+  LAKER_TRY(op.mixed_ws.ensure(round_up(nk * 8, 256) + 2 * round_up(nk * 4, 256)));
+  double* R = op.mixed_ws.as<double>();
+  float* Rf = reinterpret_cast<float*>(op.mixed_ws.as<char>() + round_up(nk * 8, 256));
+  float* D = reinterpret_cast<float*>(op.mixed_ws.as<char>() + round_up(nk * 8, 256) + round_up(nk * 4, 256));
+  DevBuf tmp;
+  const dim3 eg((unsigned)ceil_div(nk, 256)), eb(256);
+  result->outer_iterations = 0;
+  result->inner_iterations = 0;
+  result->status = LAKER_OK;
+  result->converged = 0;
+  double bnorm = 0.0;
+  LAKER_TRY(frob_norm(B, nk, tmp, &bnorm, st));
+  result->rhs_norm = bnorm;
+  LAKER_HIP_CHECK(hipMemsetAsync(X, 0, nk * 8, st));
+  if (bnorm == 0.0) {
+    result->converged = 1;
+    result->residual_norm = 0.0;
+    return LAKER_OK;
+  }
+  LAKER_HIP_CHECK(hipMemcpyAsync(R, B, nk * 8, hipMemcpyDeviceToDevice, st));
+  double rnorm = bnorm;
+  for (int outer = 0; outer < max_outer; ++outer) {
+    if (rnorm / bnorm <= tol) {
+      result->converged = 1;
+      break;
+    }
+    // inner solve A d = r / ||r|| in FP32 vectors
+    hipLaunchKernelGGL(scale_to_f32_kernel, eg, eb, 0, st, R, Rf, nk, 1.0 / rnorm);
+    LAKER_HIP_CHECK(hipGetLastError());
+    laker_pcg_result ir;
+    const double eps = std::sqrt(1.1920928955078125e-07);
+    int rc = pcg_run<float>(op, pc, Rf, k, D, nullptr, inner_tol, inner_max_iter, eps, 0,
+                            inner_precision, 8, st, &ir);
+    result->inner_iterations += ir.iterations;
+    if (rc != LAKER_OK) {
+      result->status = rc;
+      return rc;
+    }
+    hipLaunchKernelGGL(add_scaled_f32_kernel, eg, eb, 0, st, X, D, nk, rnorm);
+    LAKER_HIP_CHECK(hipGetLastError());
+    LAKER_TRY(fused_product_f64<double>(op, op.rows(), op.cols(), X, k, k, R, k, op.lambda_reg, X, k,
+                                        op.clamp_for(LAKER_CLAMP_AUTO), B, st));
+    LAKER_TRY(frob_norm(R, nk, tmp, &rnorm, st));
+    result->outer_iterations = outer + 1;
+  }
+  result->residual_norm = rnorm;
+  if (rnorm / bnorm <= tol) result->converged = 1;
+  return LAKER_OK;
+}
+
+}  // extern "C"

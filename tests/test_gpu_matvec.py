@@ -1,0 +1,156 @@
+"""Parity of the fused HIP matvec family with the FP64 oracle (GPU).
+
+Error metric: |y - y_ref| / (lambda |x| + |G| |x|) row-wise (scale of the
+summands), so cancellation in the sum does not inflate the tolerance.
+Tolerances: FP32 Gram 2e-5, BF16 hi/lo Gram ("mixed") 3e-4, FP64 1e-12.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import laker
+import laker_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp32": 2e-5, "mixed": 3e-4}
+
+
+def scale_rows(e, lam, x, clamp=None):
+    e = np.asarray(e, np.float64)
+    g = e @ e.T
+    if clamp is not None:
+        g = np.minimum(g, clamp)
+    ax = np.abs(np.asarray(x, np.float64))
+    return lam * ax + np.exp(g) @ ax
+
+
+def norm_err(y, yref, scale):
+    y = np.asarray(y, np.float64)
+    return float(np.max(np.abs(y - yref) / scale))
+
+
+def make(n, d, k, seed, amp=0.6, dtype=np.float32):
+    rng = np.random.default_rng(seed)
+    e = (rng.standard_normal((n, d)) * amp / np.sqrt(max(d, 1)) * 2).astype(dtype)
+    x = rng.standard_normal((n, k) if k > 1 else n).astype(dtype)
+    return e, x
+
+
+CASES = [(1, 1, 1), (5, 3, 1), (31, 10, 1), (64, 10, 2), (100, 6, 3), (257, 16, 4), (300, 10, 5),
+         (513, 16, 31), (700, 10, 64), (1000, 32, 70), (2000, 10, 1), (2000, 16, 1)]
+
+
+@pytest.mark.parametrize("n,d,k", CASES)
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_matvec_matches_fp64(cuda, n, d, k, precision):
+    e, x = make(n, d, k, seed=n + d + k)
+    print(f"matvec n={n} d={d} k={k} precision={precision}", flush=True)
+    op = laker.AttentionKernelOperator(torch.from_numpy(e).to(cuda), lambda_reg=0.05, precision=precision)
+    y = op.matvec(torch.from_numpy(x).to(cuda)).cpu().numpy()
+    yref = O.matvec_fp64(e, 0.05, x)
+    s = scale_rows(e, 0.05, x)
+    assert y.shape == x.shape
+    assert norm_err(y, yref, s) < TOL[precision]
+
+
+def test_matvec_fp64_operator(cuda):
+    e, x = make(400, 10, 3, seed=3, dtype=np.float64)
+    op = laker.AttentionKernelOperator(torch.from_numpy(e).to(cuda), lambda_reg=0.05)
+    y = op.matvec(torch.from_numpy(x).to(cuda)).cpu().numpy()
+    assert norm_err(y, O.matvec_fp64(e, 0.05, x), scale_rows(e, 0.05, x)) < 1e-12
+
+
+def test_matvec_reference_golden(cuda, golden):
+    g = golden("operator.npz")
+    for tag, tol in (("f32", 2e-5), ("f64", 1e-12)):
+        e = g[f"{tag}_e"]
+        op = laker.AttentionKernelOperator(torch.from_numpy(e).to(cuda), lambda_reg=0.05)
+        for xk, yk in (("x1", "y1"), ("x2", "y2")):
+            x = g[f"{tag}_{xk}"]
+            y = op.matvec(torch.from_numpy(x).to(cuda)).cpu().numpy()
+            assert norm_err(y, g[f"{tag}_{yk}"], scale_rows(e, 0.05, x)) < tol
+        np.testing.assert_allclose(op.diagonal().cpu().numpy(), g[f"{tag}_diag"], rtol=1e-6)
+        k = op.kernel_eval(torch.from_numpy(g[f"{tag}_q"]).to(cuda)).cpu().numpy()
+        np.testing.assert_allclose(k, g[f"{tag}_keval"], rtol=2e-6)
+
+
+def test_clamp_regime(cuda, golden):
+    g = golden("operator.npz")
+    e, x = g["clamp_e"], g["clamp_x1"]
+    op = laker.AttentionKernelOperator(torch.from_numpy(e).to(cuda), lambda_reg=0.05)
+    assert op.skip_clamp is False
+    y = op.matvec(torch.from_numpy(x).to(cuda)).cpu().numpy()
+    yref = O.matvec_fp64(e, 0.05, x, clamp=80.0)
+    assert norm_err(y, yref, scale_rows(e, 0.05, x, clamp=80.0)) < 2e-5
+
+
+def test_cpu_inputs_roundtrip_device(cuda):
+    e, x = make(50, 4, 1, seed=9)
+    op = laker.AttentionKernelOperator(torch.from_numpy(e), lambda_reg=0.1)
+    y = op.matvec(torch.from_numpy(x))
+    assert y.device.type == "cpu" and y.shape == (50,)
+    with pytest.raises(ValueError):
+        op.matvec(torch.zeros(50, 2, 2))
+
+
+@pytest.mark.parametrize("m,n,k", [(1, 50, 1), (17, 300, 1), (300, 1000, 2), (129, 2000, 5)])
+def test_cross_matvec(cuda, m, n, k):
+    e, x = make(n, 10, k, seed=m)
+    q, _ = make(m, 10, 1, seed=m + 1)
+    op = laker.AttentionKernelOperator(torch.from_numpy(e).to(cuda), lambda_reg=0.05)
+    y = op.cross_matvec(torch.from_numpy(q).to(cuda), torch.from_numpy(x).to(cuda)).cpu().numpy()
+    yref = O.cross_matvec_fp64(q, e, x)
+    s = np.exp(q.astype(np.float64) @ e.astype(np.float64).T) @ np.abs(x.astype(np.float64))
+    assert norm_err(y, yref, s) < 2e-5
+    # transposed: exp(E Q^T) xq
+    xq = np.random.default_rng(m).standard_normal((m, k) if k > 1 else m).astype(np.float32)
+    yt = op.cross_matvec_t(torch.from_numpy(q).to(cuda), torch.from_numpy(xq).to(cuda)).cpu().numpy()
+    yt_ref = O.cross_matvec_fp64(e, q, xq)
+    st = np.exp(e.astype(np.float64) @ q.astype(np.float64).T) @ np.abs(xq.astype(np.float64))
+    assert norm_err(yt, yt_ref, st) < 2e-5
+
+
+def test_to_dense_diagonal_kernel_eval(cuda):
+    e, _ = make(70, 5, 1, seed=2)
+    op = laker.AttentionKernelOperator(torch.from_numpy(e).to(cuda), lambda_reg=0.2)
+    a = op.to_dense().cpu().numpy()
+    ref = O.dense_matrix_fp64(e, 0.2)
+    np.testing.assert_allclose(a, ref, rtol=2e-6)
+    np.testing.assert_allclose(op.diagonal().cpu().numpy(), np.diag(ref), rtol=2e-6)
+    q, _ = make(9, 5, 1, seed=4)
+    k = op.kernel_eval(torch.from_numpy(q).to(cuda)).cpu().numpy()
+    np.testing.assert_allclose(k, np.exp(q.astype(np.float64) @ e.astype(np.float64).T), rtol=2e-6)
+
+
+def test_residual_fp64(cuda):
+    e, _ = make(900, 10, 1, seed=5)
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal((900, 2)) * 50
+    b = rng.standard_normal((900, 2))
+    op = laker.AttentionKernelOperator(torch.from_numpy(e).to(cuda), lambda_reg=0.01)
+    r = op.residual_fp64(torch.from_numpy(b).to(cuda), torch.from_numpy(x).to(cuda)).cpu().numpy()
+    rref = b - O.matvec_fp64(e, 0.01, x)
+    s = np.abs(b) + scale_rows(e, 0.01, x)
+    assert np.max(np.abs(r - rref) / s) < 1e-13
+
+
+@pytest.mark.parametrize("n", [20000, 100000])
+def test_full_size_rows_subset(cuda, n):
+    """Full-size operators (the n=100k workload): sampled rows against FP64."""
+    x_loc, y = laker.pathloss_problem(n, seed=0)
+    emb = laker.PositionEmbedding(2, 10, device=cuda)
+    with torch.no_grad():
+        e = emb(x_loc.to(cuda)).contiguous()
+    rows = np.random.default_rng(0).choice(n, 48, replace=False)
+    e64 = e.double().cpu().numpy()
+    yv = y.numpy().astype(np.float64)
+    g = np.exp(e64[rows] @ e64.T)
+    ref = 0.01 * yv[rows] + g @ yv
+    s = 0.01 * np.abs(yv[rows]) + g @ np.abs(yv)
+    for precision, tol in (("fp32", 2e-5), ("mixed", 3e-4)):
+        op = laker.AttentionKernelOperator(e, lambda_reg=0.01, precision=precision)
+        out = op.matvec(y.to(cuda)).double().cpu().numpy()[rows]
+        assert float(np.max(np.abs(out - ref) / s)) < tol, precision
+        op.close()

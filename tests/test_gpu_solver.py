@@ -1,0 +1,171 @@
+"""Device CCCP build/apply and device PCG against the reference's golden vectors
+and the FP64 oracle (GPU)."""
+
+import numpy as np
+import pytest
+import torch
+
+import laker
+import laker_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _cccp_golden(cuda, golden):
+    g = golden("cccp_pcg_f64.npz")
+    e = torch.from_numpy(g["e"]).to(cuda)
+    op = laker.AttentionKernelOperator(e, lambda_reg=1e-2)
+    pre = laker.CCCPPreconditioner(num_probes=60, gamma=0.1, max_iter=200, tol=1e-6, verbose=False,
+                                   dtype=torch.float64)
+    pre.build(op.matvec, 300, seed=5)
+    return g, op, pre
+
+
+def test_cccp_build_matches_reference(cuda, golden):
+    print("cccp golden n=300 nr=60 fp64", flush=True)
+    g, op, pre = _cccp_golden(cuda, golden)
+    assert pre.isotropic_coef == pytest.approx(float(g["isotropic_coef"]), rel=1e-9)
+    assert pre.q_basis.shape == (300, 60)
+    pv = pre.apply(torch.from_numpy(g["v"]).to(cuda)).cpu().numpy()
+    np.testing.assert_allclose(pv, g["pv"], rtol=1e-8, atol=1e-10)
+    # Q equals the reference's Householder Q up to column signs; C transforms with them
+    q = pre.q_basis.cpu().numpy()
+    sgn = np.sign(np.sum(q * g["q_basis"], axis=0))
+    np.testing.assert_allclose(q * sgn, g["q_basis"], atol=1e-8)
+    c = pre.q_basis_matrix.cpu().numpy()
+    np.testing.assert_allclose(sgn[:, None] * c * sgn[None, :], g["q_basis_matrix"], rtol=1e-7, atol=1e-10)
+    w = pre.q_eigenvalues.cpu().numpy()
+    np.testing.assert_allclose(np.sort(w), np.sort(g["q_eigenvalues"]), rtol=1e-8)
+
+
+def test_cccp_invariants_and_linearity(cuda, golden):
+    g, op, pre = _cccp_golden(cuda, golden)
+    p = pre.to_dense().cpu().numpy()
+    np.testing.assert_allclose(p, p.T, atol=1e-10)
+    q = pre.q_basis.cpu().numpy()
+    c = pre.q_basis_matrix.cpu().numpy()
+    sigma = pre.isotropic_coef * np.eye(300) + q @ c @ q.T
+    assert abs(np.trace(sigma) - 300) < 1e-8
+    w, v = np.linalg.eigh(sigma)
+    np.testing.assert_allclose(p, (v / np.sqrt(w)) @ v.T, atol=1e-8)
+    u = torch.randn(300, dtype=torch.float64, device=cuda)
+    t = torch.randn(300, dtype=torch.float64, device=cuda)
+    lhs = pre.apply(2.5 * u - 1.3 * t)
+    rhs = 2.5 * pre.apply(u) - 1.3 * pre.apply(t)
+    assert torch.allclose(lhs, rhs, rtol=1e-10, atol=1e-12)
+    k4 = torch.randn(300, 4, dtype=torch.float64, device=cuda)
+    z4 = pre.apply(k4)
+    for j in range(4):
+        assert torch.allclose(z4[:, j], pre.apply(k4[:, j].contiguous()), rtol=1e-10, atol=1e-12)
+    with pytest.raises(RuntimeError, match="not been built"):
+        laker.CCCPPreconditioner(verbose=False).apply(u)
+
+
+def test_pcg_fp64_matches_reference_trajectory(cuda, golden):
+    g, op, pre = _cccp_golden(cuda, golden)
+    pcg = laker.PreconditionedConjugateGradient(tol=1e-10, max_iter=300, verbose=False)
+    x = pcg.solve(op.matvec, pre.apply, torch.from_numpy(g["b"]).to(cuda)).cpu().numpy()
+    assert abs(pcg.iterations - int(g["iterations"])) <= 1
+    np.testing.assert_allclose(x, g["x"], rtol=1e-7, atol=1e-7)
+    pcg2 = laker.PreconditionedConjugateGradient(tol=1e-8, max_iter=300, verbose=False)
+    x2 = pcg2.solve(op.matvec, pre.apply, torch.from_numpy(g["b2"]).to(cuda)).cpu().numpy()
+    assert abs(pcg2.iterations - int(g["iterations2"])) <= 1
+    np.testing.assert_allclose(x2, g["x2"], rtol=1e-6, atol=1e-6)
+
+
+def test_pcg_edge_cases(cuda):
+    e = torch.randn(64, 4, generator=torch.Generator().manual_seed(0)) * 0.5
+    op = laker.AttentionKernelOperator(e.to(cuda), lambda_reg=0.1)
+    pre = laker.CCCPPreconditioner(num_probes=32, verbose=False).build(op.matvec, 64, seed=1)
+    # zero RHS returns zeros without iterating
+    pcg = laker.PreconditionedConjugateGradient(tol=1e-8, max_iter=50, verbose=False)
+    x = pcg.solve(op.matvec, pre.apply, torch.zeros(64, device=cuda))
+    assert float(x.abs().sum()) == 0.0 and pcg.iterations == 0
+    # max_iter=1 forces non-convergence
+    b = torch.randn(64, device=cuda)
+    pcg1 = laker.PreconditionedConjugateGradient(tol=1e-12, max_iter=1, verbose=False)
+    pcg1.solve(op.matvec, pre.apply, b)
+    assert pcg1.iterations == 1 and not pcg1.converged
+    # bad dimensions
+    with pytest.raises(ValueError):
+        pcg.solve(op.matvec, pre.apply, torch.zeros(4, 4, 4, device=cuda))
+    # indefinite operator: lambda << 0 forces the breakdown test
+    bad = laker.AttentionKernelOperator(e.to(cuda), lambda_reg=-1e4)
+    with pytest.raises(RuntimeError, match="breakdown"):
+        laker.PreconditionedConjugateGradient(tol=1e-8, max_iter=50, verbose=False).solve(bad.matvec, None, b)
+    # warm start from the solution converges immediately-ish
+    xs = laker.PreconditionedConjugateGradient(tol=1e-6, max_iter=200, verbose=False).solve(op.matvec, pre.apply, b)
+    pw = laker.PreconditionedConjugateGradient(tol=1e-6, max_iter=200, verbose=False)
+    pw.solve(op.matvec, pre.apply, b, x0=xs)
+    assert pw.iterations <= 3
+
+
+def test_pcg_generic_callables(cuda):
+    n = 20
+    a = torch.diag(torch.arange(1, n + 1, dtype=torch.float64, device=cuda))
+    b = torch.randn(n, dtype=torch.float64, device=cuda)
+    pcg = laker.PreconditionedConjugateGradient(tol=1e-12, max_iter=n, verbose=False)
+    x = pcg.solve(lambda v: a @ v, lambda v: torch.linalg.solve(a, v), b)
+    assert pcg.iterations == 1
+    assert torch.allclose(x, b / torch.arange(1, n + 1, device=cuda), rtol=1e-10)
+    # generic operator, native preconditioner build from responses
+    e = torch.randn(80, 4, dtype=torch.float64, generator=torch.Generator().manual_seed(2)) * 0.5
+    op = laker.AttentionKernelOperator(e.to(cuda), lambda_reg=1e-2)
+    dense = op.to_dense()
+    pre = laker.CCCPPreconditioner(num_probes=40, max_iter=30, tol=1e-5, verbose=False,
+                                   dtype=torch.float64).build(lambda v: dense @ v, 80, seed=3)
+    native = laker.CCCPPreconditioner(num_probes=40, max_iter=30, tol=1e-5, verbose=False,
+                                      dtype=torch.float64).build(op.matvec, 80, seed=3)
+    assert pre.isotropic_coef == pytest.approx(native.isotropic_coef, rel=1e-10)
+    bb = torch.randn(80, dtype=torch.float64, device=cuda)
+    x = laker.PreconditionedConjugateGradient(tol=1e-10, max_iter=200, verbose=False).solve(
+        lambda v: dense @ v, pre.apply, bb)
+    assert float(torch.linalg.norm(dense @ x - bb) / torch.linalg.norm(bb)) < 1e-8
+
+
+def test_fit_n2000_against_reference_and_fp64(cuda, golden):
+    g = golden("fit_n2000.npz")
+    e = torch.from_numpy(g["embeddings"]).to(cuda)
+    y = torch.from_numpy(g["y"]).to(cuda)
+    op = laker.AttentionKernelOperator(e, lambda_reg=1e-2)
+    pre = laker.CCCPPreconditioner(gamma=0.1, verbose=False).build(op.matvec, 2000)
+    assert pre.nr == 200
+    # the reference's CCCP runs in fp32; this build's loop is FP64
+    assert pre.isotropic_coef == pytest.approx(float(g["isotropic_coef"]), rel=1e-4)
+    pcg = laker.PreconditionedConjugateGradient(tol=1e-6, max_iter=1000, verbose=False)
+    alpha = pcg.solve(op.matvec, pre.apply, y).double().cpu().numpy()
+    ref_it = int(g["iterations"])
+    print(f"fp32 PCG iterations {pcg.iterations} (reference {ref_it})", flush=True)
+    assert abs(pcg.iterations - ref_it) <= 0.15 * ref_it
+    a64 = O.dense_solve_fp64(g["embeddings"], 1e-2, g["y"].astype(np.float64))
+    err = np.linalg.norm(alpha - a64) / np.linalg.norm(a64)
+    err_ref = np.linalg.norm(g["alpha"] - a64) / np.linalg.norm(a64)
+    print(f"alpha error vs FP64: HIP fp32 {err:.2e}, reference fp32 {err_ref:.2e}", flush=True)
+    assert err < 3 * err_ref
+    # mixed precision: BF16-Gram inner PCG + FP64 refinement reaches the FP64 solution
+    pm = laker.PreconditionedConjugateGradient(tol=1e-7, max_iter=1000, verbose=False, precision="mixed")
+    pm.solve(op.matvec, pre.apply, y)
+    am = pm.solution_fp64.cpu().numpy()
+    err_m = np.linalg.norm(am - a64) / np.linalg.norm(a64)
+    print(f"mixed: outer {pm.outer_iterations} inner {pm.iterations} fp64 res {pm.residual_norm:.2e} "
+          f"error {err_m:.2e}", flush=True)
+    assert pm.converged
+    assert err_m < 1e-6
+    a = O.dense_matrix_fp64(g["embeddings"], 1e-2)
+    assert np.linalg.norm(a @ am - g["y"]) / np.linalg.norm(g["y"]) <= 1.0000001e-7
+
+
+def test_multi_rhs_frobenius_rule(cuda, golden):
+    g = golden("fit_n2000.npz")
+    e = torch.from_numpy(g["embeddings"]).to(cuda)
+    op = laker.AttentionKernelOperator(e, lambda_reg=1e-2)
+    pre = laker.CCCPPreconditioner(gamma=0.1, verbose=False).build(op.matvec, 2000)
+    b = torch.randn(2000, 6, generator=torch.Generator().manual_seed(0)).to(cuda)
+    pcg = laker.PreconditionedConjugateGradient(tol=1e-5, max_iter=1000, verbose=False)
+    x = pcg.solve(op.matvec, pre.apply, b)
+    r = b.double() - op.matvec(x).double()
+    assert float(torch.linalg.norm(r) / torch.linalg.norm(b.double())) < 5e-5
+    # each column alone solved with the same rule gives the same answer to solver accuracy
+    x0 = laker.PreconditionedConjugateGradient(tol=1e-5, max_iter=1000, verbose=False).solve(
+        op.matvec, pre.apply, b[:, 0].contiguous())
+    assert float(torch.linalg.norm(x[:, 0] - x0) / torch.linalg.norm(x0)) < 1e-2
