@@ -72,13 +72,16 @@ int laker_matvec_backward(laker_operator_t h, const float* X, const float* dY, i
     const double* E64 = op.self.e64.as<double>();
     const dim3 g((unsigned)ceil_div(n * kd, 256)), b(256);
     hipLaunchKernelGGL(outer_rhs_kernel, g, b, 0, st, X, E64, n, k, d, U, w, 0);
+    LAKER_KCHECK("outer_rhs_kernel");
     hipLaunchKernelGGL(outer_rhs_kernel, g, b, 0, st, dY, E64, n, k, d, U, w, kd);
+    LAKER_KCHECK("outer_rhs_kernel");
     LAKER_HIP_CHECK(hipGetLastError());
     LAKER_TRY(fused_product_f32(op, LAKER_PREC_FP32, op.rows(), op.cols(), U, (int)w, w, W, w, 0.0,
                                 nullptr, 0, clamp, true, 0, st));
     // W[:, 0:kd] = G'(X o E), W[:, kd:2kd] = G'(dY o E)
     hipLaunchKernelGGL(contract_grad_kernel, dim3((unsigned)ceil_div(n * d, 256)), b, 0, st, dY, X, W, w, 0,
                        kd, n, k, d, op.scale, dE, 0);
+    LAKER_KCHECK("contract_grad_kernel");
     LAKER_HIP_CHECK(hipGetLastError());
   }
   return LAKER_OK;
@@ -115,22 +118,26 @@ int laker_cross_matvec_backward(laker_operator_t h, const float* Eq, int64_t m, 
     LAKER_CHECK_ARG(X != nullptr, "X is required for the query-embedding gradient");
     hipLaunchKernelGGL(outer_rhs_kernel, dim3((unsigned)ceil_div(n * kd, 256)), b, 0, st, X,
                        op.self.e64.as<double>(), n, k, d, U, (int64_t)kd, 0);
+    LAKER_KCHECK("outer_rhs_kernel");
     LAKER_HIP_CHECK(hipGetLastError());
     LAKER_TRY(fused_product_f32(op, LAKER_PREC_FP32, qrows, op.cols(), U, kd, kd, Wq, kd, 0.0, nullptr, 0,
                                 clamp, true, 0, st));
     hipLaunchKernelGGL(contract_grad_kernel, dim3((unsigned)ceil_div(m * d, 256)), b, 0, st, dYq,
                        (const float*)nullptr, Wq, (int64_t)kd, 0, 0, m, k, d, op.scale, dEq, 1);
+    LAKER_KCHECK("contract_grad_kernel");
     LAKER_HIP_CHECK(hipGetLastError());
   }
   if (dE) {
     LAKER_CHECK_ARG(X != nullptr, "X is required for the embedding gradient");
     hipLaunchKernelGGL(outer_rhs_kernel, dim3((unsigned)ceil_div(m * kd, 256)), b, 0, st, dYq,
                        op.query.e64.as<double>(), m, k, d, V, (int64_t)kd, 0);
+    LAKER_KCHECK("outer_rhs_kernel");
     LAKER_HIP_CHECK(hipGetLastError());
     LAKER_TRY(fused_product_f32(op, LAKER_PREC_FP32, op.rows(), qcols, V, kd, kd, Wn, kd, 0.0, nullptr, 0,
                                 clamp, true, 0, st));
     hipLaunchKernelGGL(contract_grad_kernel, dim3((unsigned)ceil_div(n * d, 256)), b, 0, st, X,
                        (const float*)nullptr, Wn, (int64_t)kd, 0, 0, n, k, d, op.scale, dE, 1);
+    LAKER_KCHECK("contract_grad_kernel");
     LAKER_HIP_CHECK(hipGetLastError());
   }
   return LAKER_OK;

@@ -306,13 +306,16 @@ static int apply_t(Precond& pc, const T* X, int k, int64_t ldx, T* Z, int64_t ld
     LAKER_TRY(pc.part.ensure((size_t)nb_eff * nr * 8));
     hipLaunchKernelGGL((qtx_vec_kernel<T>), dim3((unsigned)nb_eff), dim3(256), 0, st, Q, X, n, nr, rpb,
                        pc.part.as<double>());
+    LAKER_KCHECK("qtx_vec_kernel");
     hipLaunchKernelGGL(reduce_parts_cols_kernel, dim3((unsigned)ceil_div(nr, 256)), dim3(256), 0, st,
                        pc.part.as<double>(), (int)nb_eff, nr, W1);
+    LAKER_KCHECK("reduce_parts_cols_kernel");
     LAKER_HIP_CHECK(hipGetLastError());
     LAKER_TRY((gemm<double, double, double>(false, false, nr, 1, nr, 1.0, pc.S.as<double>(), nr, W1,
                                              1, 0.0, W2, 1, nullptr, st)));
     hipLaunchKernelGGL((qw_vec_kernel<T>), dim3((unsigned)ceil_div(n, 4)), dim3(256), nr * sizeof(double),
                        st, Q, W2, X, Z, n, nr, pc.inv_sqrt_a);
+    LAKER_KCHECK("qw_vec_kernel");
     LAKER_HIP_CHECK(hipGetLastError());
     return LAKER_OK;
   }
@@ -322,6 +325,7 @@ static int apply_t(Precond& pc, const T* X, int k, int64_t ldx, T* Z, int64_t ld
                                            0.0, W2, k, nullptr, st)));
   hipLaunchKernelGGL((scale_copy_kernel<T>), dim3((unsigned)ceil_div(n * k, 256)), dim3(256), 0, st, X,
                      ldx, Z, ldz, n, k, pc.inv_sqrt_a);
+  LAKER_KCHECK("scale_copy_kernel");
   LAKER_HIP_CHECK(hipGetLastError());
   LAKER_TRY((gemm<T, double, T>(false, false, n, k, nr, 1.0, Q, nr, W2, k, 1.0, Z, ldz, nullptr, st)));
   return LAKER_OK;
@@ -347,6 +351,7 @@ static int cholqr_pass(double* V, int64_t n, int nr, double shift, double* G, do
   if (shift > 0.0) {
     hipLaunchKernelGGL(add_diag_kernel, dim3((unsigned)ceil_div(nr, 256)), dim3(256), 0, st, G, (int64_t)nr,
                        shift);
+    LAKER_KCHECK("add_diag_kernel");
   }
   LAKER_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int), st));
   LAKER_TRY(potrf_lower(G, nr, nr, info, ws, st));
@@ -380,12 +385,16 @@ static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, i
   LAKER_TRY(ss.ensure((size_t)nr * 8));
   if (pc->dtype == LAKER_F32) {
     hipLaunchKernelGGL((colsumsq_kernel<float>), dim3(nr), dim3(256), 0, st, U.as<float>(), n, nr, ss.as<double>());
+    LAKER_KCHECK("colsumsq_kernel");
     hipLaunchKernelGGL((colscale_kernel<float>), dim3((unsigned)ceil_div(n * nr, 256)), dim3(256), 0, st,
                        U.as<float>(), n, nr, ss.as<double>(), epsilon);
+    LAKER_KCHECK("colscale_kernel");
   } else {
     hipLaunchKernelGGL((colsumsq_kernel<double>), dim3(nr), dim3(256), 0, st, U.as<double>(), n, nr, ss.as<double>());
+    LAKER_KCHECK("colsumsq_kernel");
     hipLaunchKernelGGL((colscale_kernel<double>), dim3((unsigned)ceil_div(n * nr, 256)), dim3(256), 0, st,
                        U.as<double>(), n, nr, ss.as<double>(), epsilon);
+    LAKER_KCHECK("colscale_kernel");
   }
   LAKER_HIP_CHECK(hipGetLastError());
 
@@ -469,21 +478,26 @@ static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, i
   for (int it = 0; it < max_iter; ++it) {
     hipLaunchKernelGGL(form_m_kernel, dim3((unsigned)ceil_div(nn, 256)), dim3(256), 0, st, Cb.as<double>(),
                        Mb.as<double>(), (int64_t)nr, a);
+    LAKER_KCHECK("form_m_kernel");
     LAKER_HIP_CHECK(hipMemsetAsync(infob.ptr, 0, sizeof(int), st));
     LAKER_TRY(potrf_lower(Mb.as<double>(), nr, nr, infob.as<int>(), &pc->ws, st));
     LAKER_HIP_CHECK(hipMemcpyAsync(Zb.ptr, Rq, nn * 8, hipMemcpyDeviceToDevice, st));
     LAKER_TRY(trsm_lower_left(Mb.as<double>(), nr, nr, Zb.as<double>(), nr, nr, &pc->ws, st));
     hipLaunchKernelGGL(cccp_weights_kernel, dim3(nr), dim3(256), 0, st, Zb.as<double>(), Rq, Rw.as<double>(),
                        nr, n_over_nr, epsilon);
+    LAKER_KCHECK("cccp_weights_kernel");
     LAKER_TRY((gemm<double, double, double>(false, true, nr, nr, nr, 1.0, Rw.as<double>(), nr, Rq, nr, 0.0,
                                              Hb.as<double>(), nr, nullptr, st)));
     hipLaunchKernelGGL(cccp_trace_kernel, dim3(1), dim3(256), 0, st, Hb.as<double>(), nr, gamma, s, rho,
                        a_rho, n, scb.as<CccpScalars>());
+    LAKER_KCHECK("cccp_trace_kernel");
     hipLaunchKernelGGL(cccp_update_kernel, dim3(upd_blocks), dim3(256), 0, st, Hb.as<double>(),
                        Cb.as<double>(), Cn.as<double>(), nr, gamma, s, rho, scb.as<CccpScalars>(),
                        partb.as<double>());
+    LAKER_KCHECK("cccp_update_kernel");
     hipLaunchKernelGGL(cccp_norms_final_kernel, dim3(1), dim3(64), 0, st, partb.as<double>(), upd_blocks,
                        scb.as<CccpScalars>());
+    LAKER_KCHECK("cccp_norms_final_kernel");
     LAKER_HIP_CHECK(hipGetLastError());
     CccpScalars hs;
     int hinfo = 0;
@@ -492,7 +506,7 @@ static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, i
     if (hinfo != 0) return fail(LAKER_ENUMERIC, "CCCP build: M = aI + C is not positive definite");
     const double a_prev = a;
     a = hs.a_new;
-    std::swap(Cb, Cn);
+    Cb.swap(Cn);
     const double iso_rel = std::fabs(a - a_prev) / (std::fabs(a_prev) + 1e-12);
     const double q_rel = std::sqrt(hs.diff2) / (std::sqrt(hs.c2) + 1e-12);
     it_done = it + 1;
@@ -506,7 +520,9 @@ static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, i
   // 5. S = M^{-1/2} - a^{-1/2} I by coupled Newton-Schulz on M / c
   hipLaunchKernelGGL(form_m_kernel, dim3((unsigned)ceil_div(nn, 256)), dim3(256), 0, st, Cb.as<double>(),
                      Mb.as<double>(), (int64_t)nr, a);
+  LAKER_KCHECK("form_m_kernel");
   hipLaunchKernelGGL(frob2_kernel, dim3(1), dim3(1024), 0, st, Mb.as<double>(), nn, scb.as<double>());
+  LAKER_KCHECK("frob2_kernel");
   LAKER_HIP_CHECK(hipGetLastError());
   double fro2 = 0.0;
   LAKER_TRY(read_scalar(scb.ptr, &fro2, 8, st));
@@ -514,6 +530,7 @@ static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, i
   // Y = M / c  (in Mb), Z = I (in Zb)
   hipLaunchKernelGGL(scale_sub_diag_kernel, dim3((unsigned)ceil_div(nn, 256)), dim3(256), 0, st,
                      Mb.as<double>(), Mb.as<double>(), (int64_t)nr, 1.0 / cn, 0.0);
+  LAKER_KCHECK("scale_sub_diag_kernel");
   LAKER_TRY(set_identity(Zb.as<double>(), nr, 1.0, st));
   const int ns_blocks = (int)std::min<int64_t>(256, ceil_div(nn, 256));
   LAKER_TRY(partb.ensure((size_t)std::max(ns_blocks, upd_blocks) * 2 * 8));
@@ -525,8 +542,10 @@ static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, i
                                              Mb.as<double>(), nr, 0.0, Hb.as<double>(), nr, nullptr, st)));
     hipLaunchKernelGGL(ns_t_kernel, dim3(ns_blocks), dim3(256), 0, st, Hb.as<double>(), Rw.as<double>(),
                        (int64_t)nr, partb.as<double>());
+    LAKER_KCHECK("ns_t_kernel");
     hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(64), 0, st, partb.as<double>(), ns_blocks,
                        scb.as<double>());
+    LAKER_KCHECK("sum_parts_kernel");
     LAKER_HIP_CHECK(hipGetLastError());
     double err2 = 0.0;
     LAKER_TRY(read_scalar(scb.ptr, &err2, 8, st));
@@ -549,6 +568,7 @@ static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, i
   // S = (Z + Z^T)/(2 sqrt(c)) - a^{-1/2} I
   hipLaunchKernelGGL(sym_scale_kernel, dim3((unsigned)ceil_div(nn, 256)), dim3(256), 0, st, Zb.as<double>(),
                      pc->S.as<double>(), (int64_t)nr, 1.0 / std::sqrt(cn), pc->inv_sqrt_a);
+  LAKER_KCHECK("sym_scale_kernel");
   LAKER_HIP_CHECK(hipGetLastError());
   LAKER_HIP_CHECK(hipStreamSynchronize(st));
   if (iterations) *iterations = it_done;
@@ -590,7 +610,7 @@ int laker_precond_build(laker_operator_t oh, int32_t nr, double gamma, double ep
     else
       hipLaunchKernelGGL((randn_kernel<double>), dim3((unsigned)ceil_div(n * nr, 256)), dim3(256), 0, st,
                          probe_buf.as<double>(), n * nr, seed);
-    LAKER_HIP_CHECK(hipGetLastError());
+    LAKER_KCHECK("randn_kernel");
     Rp = probe_buf.ptr;
   }
   LAKER_TRY(U.ensure((size_t)n * nr * esz));

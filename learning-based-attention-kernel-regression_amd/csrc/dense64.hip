@@ -131,6 +131,7 @@ int gemm(bool tra, bool trb, int64_t M, int64_t N, int64_t K, double alpha, cons
   if (ks > 1) {
     hipLaunchKernelGGL((gemm_splitk_reduce<TC>), dim3((unsigned)ceil_div(M * N, 256)), dim3(256), 0,
                        st, part, M, N, ks, alpha, beta, C, ldc);
+    LAKER_KCHECK("gemm_splitk_reduce");
     LAKER_HIP_CHECK(hipGetLastError());
   }
   return LAKER_OK;
@@ -234,10 +235,12 @@ int potrf_lower(double* A, int64_t lda, int64_t n, int* info_dev, DevBuf* ws, hi
   for (int64_t k0 = 0; k0 < n; k0 += PNB) {
     const int nb = (int)std::min<int64_t>(PNB, n - k0);
     hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, st, A, lda, k0, nb, info_dev);
+    LAKER_KCHECK("potrf_diag_kernel");
     const int64_t rem = n - k0 - nb;
     if (rem > 0) {
       hipLaunchKernelGGL(potrf_panel_kernel, dim3((unsigned)ceil_div(rem, 256)), dim3(256), 0, st, A,
                          lda, n, k0, nb);
+      LAKER_KCHECK("potrf_panel_kernel");
       // A22 -= L21 L21^T  (full square of the trailing block; upper part is scratch)
       double* L21 = A + (k0 + nb) * lda + k0;
       double* A22 = A + (k0 + nb) * lda + k0 + nb;
@@ -247,6 +250,7 @@ int potrf_lower(double* A, int64_t lda, int64_t n, int* info_dev, DevBuf* ws, hi
   }
   hipLaunchKernelGGL(zero_upper_kernel, dim3((unsigned)ceil_div(n * n, 256)), dim3(256), 0, st, A,
                      lda, n);
+  LAKER_KCHECK("zero_upper_kernel");
   LAKER_HIP_CHECK(hipGetLastError());
   return LAKER_OK;
 }
@@ -290,6 +294,7 @@ int trsm_lower_left(const double* L, int64_t ldl, int64_t n, double* B, int64_t 
     const int nb = (int)std::min<int64_t>(PNB, n - k0);
     hipLaunchKernelGGL(trsm_diag_kernel, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, st, L, ldl,
                        B, ldb, m, k0, nb);
+    LAKER_KCHECK("trsm_diag_kernel");
     const int64_t rem = n - k0 - nb;
     if (rem > 0) {
       // B[k0+nb:, :] -= L[k0+nb:, k0:k0+nb] * B[k0:k0+nb, :]
@@ -314,6 +319,7 @@ __global__ void set_identity_kernel(double* __restrict__ A, int64_t n, double di
 int set_identity(double* A, int64_t n, double diag, hipStream_t st) {
   hipLaunchKernelGGL(set_identity_kernel, dim3((unsigned)ceil_div(n * n, 256)), dim3(256), 0, st, A,
                      n, diag);
+  LAKER_KCHECK("set_identity_kernel");
   LAKER_HIP_CHECK(hipGetLastError());
   return LAKER_OK;
 }

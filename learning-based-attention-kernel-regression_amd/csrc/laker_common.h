@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
+#include <utility>
 
 #include "../../include/laker_hip.h"
 
@@ -27,6 +28,16 @@ int fail(int code, const std::string& msg);
     if (_e != hipSuccess)                                                              \
       return ::laker::fail(LAKER_EHIP, std::string("HIP error: ") + hipGetErrorString(_e) + \
                                            " at " __FILE__ ":" + std::to_string(__LINE__)); \
+  } while (0)
+
+// Status of the kernel launch just issued, named for the error text.
+#define LAKER_KCHECK(name)                                                               \
+  do {                                                                                   \
+    hipError_t _e = hipGetLastError();                                                   \
+    if (_e != hipSuccess)                                                                \
+      return ::laker::fail(LAKER_EHIP, std::string("HIP error launching ") + (name) + ": " + \
+                                           hipGetErrorString(_e) + " at " __FILE__ ":" +  \
+                                           std::to_string(__LINE__));                    \
   } while (0)
 
 #define LAKER_CHECK_ARG(cond, msg)                                \
@@ -58,6 +69,27 @@ __device__ __host__ inline float bf16_bits_to_f32(uint16_t b) {
 struct DevBuf {
   void* ptr = nullptr;
   size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), bytes(o.bytes) {
+    o.ptr = nullptr;
+    o.bytes = 0;
+  }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      ptr = o.ptr;
+      bytes = o.bytes;
+      o.ptr = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
+  void swap(DevBuf& o) noexcept {
+    std::swap(ptr, o.ptr);
+    std::swap(bytes, o.bytes);
+  }
   int ensure(size_t want);
   void release();
   ~DevBuf() { release(); }

@@ -642,6 +642,7 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
                        dim3((unsigned)ceil_div(tot, 256)), dim3(256), 0, st, part, n_rows, k,
                        (int64_t)kcp, jsplit, lam, Xr, ldxr, (const float*)nullptr, Y, ldy,
                        accumulate);
+    LAKER_KCHECK("mv_reduce_kernel");
     LAKER_HIP_CHECK(hipGetLastError());
     return LAKER_OK;
   }
@@ -662,6 +663,7 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
   hipLaunchKernelGGL((mv_reduce_kernel<float, float, float, false>),
                      dim3((unsigned)ceil_div(tot, 256)), dim3(256), 0, st, part, n_rows, k, kp,
                      jsplit, lam, Xr, ldxr, (const float*)nullptr, Y, ldy, accumulate);
+  LAKER_KCHECK("mv_reduce_kernel");
   LAKER_HIP_CHECK(hipGetLastError());
   return LAKER_OK;
 }
@@ -686,10 +688,12 @@ int fused_product_f64(Operator& op, const Side& rows, const Side& cols, const XT
     hipLaunchKernelGGL((mv_reduce_kernel<double, XT, double, true>),
                        dim3((unsigned)ceil_div(tot, 256)), dim3(256), 0, st, part, n_rows, k, ldp,
                        jsplit, lam, Xr, ldxr, B, Y, ldy, 0);
+    LAKER_KCHECK("mv_reduce_kernel");
   } else {
     hipLaunchKernelGGL((mv_reduce_kernel<double, XT, double, false>),
                        dim3((unsigned)ceil_div(tot, 256)), dim3(256), 0, st, part, n_rows, k, ldp,
                        jsplit, lam, Xr, ldxr, (const double*)nullptr, Y, ldy, 0);
+    LAKER_KCHECK("mv_reduce_kernel");
   }
   LAKER_HIP_CHECK(hipGetLastError());
   return LAKER_OK;
@@ -722,12 +726,15 @@ static int pack_store_t(SideStore& s, const TE* E, int64_t n, int d, int ka32, i
   hipLaunchKernelGGL((pack_f32_kernel<TE>), dim3((unsigned)ceil_div(n32, 256)), dim3(256), 0, st, E,
                      n, d, s.npad, ka32, (float)cscale, need_col ? s.col32.as<float>() : nullptr,
                      need_row ? s.row32.as<float>() : nullptr);
+  LAKER_KCHECK("pack_f32_kernel");
   hipLaunchKernelGGL((pack_bf16x4_kernel<TE>), dim3((unsigned)ceil_div(n16, 256)), dim3(256), 0, st,
                      E, n, d, s.npad, ka16, (float)cscale,
                      need_col ? s.col16.as<uint16_t>() : nullptr,
                      need_row ? s.row16.as<uint16_t>() : nullptr);
+  LAKER_KCHECK("pack_bf16x4_kernel");
   hipLaunchKernelGGL((pack_f64_kernel<TE>), dim3((unsigned)ceil_div(n * d, 256)), dim3(256), 0, st,
                      E, n, d, s.e64.as<double>());
+  LAKER_KCHECK("pack_f64_kernel");
   LAKER_HIP_CHECK(hipGetLastError());
   return LAKER_OK;
 }
@@ -899,7 +906,7 @@ int laker_kernel_eval(const void* X, int64_t m, const void* Y, int64_t p, int32_
   else
     hipLaunchKernelGGL((kernel_eval_kernel<double>), grid, dim3(256), 0, stream, (const double*)X, m,
                        (const double*)Y, p, d, scale, clamp, 700.0, 0.0, (double*)K, ldk);
-  LAKER_HIP_CHECK(hipGetLastError());
+  LAKER_KCHECK("kernel_eval_kernel");
   return LAKER_OK;
 }
 
@@ -913,7 +920,7 @@ int laker_diagonal(laker_operator_t h, void* out, hipStream_t stream) {
     hipLaunchKernelGGL((diag_kernel<double>), dim3((unsigned)ceil_div(op->n, 256)), dim3(256), 0,
                        stream, op->self.e64.as<double>(), op->n, op->d, op->scale, op->lambda_reg,
                        (double*)out);
-  LAKER_HIP_CHECK(hipGetLastError());
+  LAKER_KCHECK("diag_kernel");
   return LAKER_OK;
 }
 
@@ -929,6 +936,7 @@ int laker_to_dense(laker_operator_t h, void* out, hipStream_t stream) {
     hipLaunchKernelGGL((kernel_eval_kernel<double>), grid, dim3(256), 0, stream,
                        op->self.e64.as<double>(), op->n, op->self.e64.as<double>(), op->n, op->d,
                        op->scale, 0, 700.0, op->lambda_reg, (double*)out, op->n);
+  LAKER_KCHECK("kernel_eval_kernel");
   LAKER_HIP_CHECK(hipGetLastError());
   return LAKER_OK;
 }

@@ -236,7 +236,7 @@ static int coldots(const PcgBuffers<T>& b, int nq, const T* a0, const T* b0, con
   else
     hipLaunchKernelGGL((coldots_kernel<T, 3>), dim3(b.nblk), dim3(256), 0, st, a0, b0, a1, b1, a2, b2, n,
                        k, b.rpb, b.part, b.nblk, s);
-  LAKER_HIP_CHECK(hipGetLastError());
+  LAKER_KCHECK("coldots_kernel");
   return LAKER_OK;
 }
 
@@ -294,19 +294,24 @@ static int pcg_run(Operator& op, Precond* pc, const T* B, int k, T* X, const T* 
   // set-up: x = x0 or 0 ; r = b - A x0 or b ; z = P r ; p = z
   if (X0) {
     hipLaunchKernelGGL((copy_kernel<T>), eg, eb, 0, st, X0, b.x, nk);
+    LAKER_KCHECK("copy_kernel");
     LAKER_TRY(op_apply<T>(op, precision, b.x, k, b.tmp, nullptr, st));
     hipLaunchKernelGGL((axpby_kernel<T>), eg, eb, 0, st, B, 1.0, (const T*)b.tmp, -1.0, b.r, nk,
                        (const PcgState*)nullptr);
+    LAKER_KCHECK("axpby_kernel");
   } else {
     LAKER_HIP_CHECK(hipMemsetAsync(b.x, 0, nk * sizeof(T), st));
     hipLaunchKernelGGL((copy_kernel<T>), eg, eb, 0, st, B, b.r, nk);
+    LAKER_KCHECK("copy_kernel");
   }
   LAKER_HIP_CHECK(hipGetLastError());
   if (pc) LAKER_TRY(precond_apply(*pc, b.r, k, k, b.z, k, st));
   else hipLaunchKernelGGL((copy_kernel<T>), eg, eb, 0, st, (const T*)b.r, b.z, nk);
   hipLaunchKernelGGL((copy_kernel<T>), eg, eb, 0, st, (const T*)b.z, b.p, nk);
+  LAKER_KCHECK("copy_kernel");
   LAKER_TRY(coldots<T>(b, 2, b.r, b.z, B, B, nullptr, nullptr, n, k, false, st));
   hipLaunchKernelGGL(pcg_setup_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.st, tol, eps);
+  LAKER_KCHECK("pcg_setup_kernel");
   LAKER_HIP_CHECK(hipGetLastError());
   PcgState hs;
   LAKER_HIP_CHECK(hipMemcpyAsync(&hs, b.st, sizeof(hs), hipMemcpyDeviceToHost, st));
@@ -333,21 +338,27 @@ static int pcg_run(Operator& op, Precond* pc, const T* B, int k, T* X, const T* 
     LAKER_TRY(op_apply<T>(op, precision, b.p, k, b.ap, &b.st->done, st));
     LAKER_TRY(coldots<T>(b, 3, b.p, b.ap, b.p, b.p, b.ap, b.ap, n, k, true, st));
     hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.alpha, b.st);
+    LAKER_KCHECK("pcg_alpha_kernel");
     hipLaunchKernelGGL((pcg_update_xr_kernel<T>), eg, eb, 0, st, b.x, b.r, b.p, b.ap, b.alpha, n, k, b.st);
+    LAKER_KCHECK("pcg_update_xr_kernel");
     LAKER_HIP_CHECK(hipGetLastError());
     if (restart_freq > 0 && (it + 1) % restart_freq == 0) {
       // residual replacement r = b - A x
       LAKER_TRY(op_apply<T>(op, precision, b.x, k, b.tmp, &b.st->done, st));
       hipLaunchKernelGGL((axpby_kernel<T>), eg, eb, 0, st, B, 1.0, (const T*)b.tmp, -1.0, b.r, nk,
                          (const PcgState*)b.st);
+      LAKER_KCHECK("axpby_kernel");
     }
     LAKER_TRY(coldots<T>(b, 1, b.r, b.r, nullptr, nullptr, nullptr, nullptr, n, k, true, st));
     hipLaunchKernelGGL(pcg_resid_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, it, max_iter, b.st);
+    LAKER_KCHECK("pcg_resid_kernel");
     if (pc) LAKER_TRY(precond_apply(*pc, b.r, k, k, b.z, k, st));
     else hipLaunchKernelGGL((copy_kernel<T>), eg, eb, 0, st, (const T*)b.r, b.z, nk);
     LAKER_TRY(coldots<T>(b, 1, b.r, b.z, nullptr, nullptr, nullptr, nullptr, n, k, true, st));
     hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.beta, b.st);
+    LAKER_KCHECK("pcg_beta_kernel");
     hipLaunchKernelGGL((pcg_update_p_kernel<T>), eg, eb, 0, st, b.p, b.z, b.beta, n, k, b.st);
+    LAKER_KCHECK("pcg_update_p_kernel");
     LAKER_HIP_CHECK(hipGetLastError());
     if ((it + 1) % check_every == 0 || it + 1 == max_iter) {
       LAKER_HIP_CHECK(hipMemcpyAsync(&hs, b.st, sizeof(hs), hipMemcpyDeviceToHost, st));
@@ -387,6 +398,7 @@ static int frob_norm(const double* a, int64_t count, DevBuf& tmp, double* out, h
   const int nb = (int)std::min<int64_t>(256, ceil_div(count, 256));
   LAKER_TRY(tmp.ensure((size_t)nb * 8));
   hipLaunchKernelGGL(frob2_part_kernel, dim3(nb), dim3(256), 0, st, a, count, tmp.as<double>());
+  LAKER_KCHECK("frob2_part_kernel");
   LAKER_HIP_CHECK(hipGetLastError());
   std::vector<double> h(nb);
   LAKER_HIP_CHECK(hipMemcpyAsync(h.data(), tmp.ptr, nb * 8, hipMemcpyDeviceToHost, st));
@@ -472,6 +484,7 @@ int laker_solve_mixed(laker_operator_t oh, laker_precond_t ph, const double* B, 
     }
     // inner solve A d = r / ||r|| in FP32 vectors
     hipLaunchKernelGGL(scale_to_f32_kernel, eg, eb, 0, st, R, Rf, nk, 1.0 / rnorm);
+    LAKER_KCHECK("scale_to_f32_kernel");
     LAKER_HIP_CHECK(hipGetLastError());
     laker_pcg_result ir;
     const double eps = std::sqrt(1.1920928955078125e-07);
@@ -483,6 +496,7 @@ int laker_solve_mixed(laker_operator_t oh, laker_precond_t ph, const double* B, 
       return rc;
     }
     hipLaunchKernelGGL(add_scaled_f32_kernel, eg, eb, 0, st, X, D, nk, rnorm);
+    LAKER_KCHECK("add_scaled_f32_kernel");
     LAKER_HIP_CHECK(hipGetLastError());
     LAKER_TRY(fused_product_f64<double>(op, op.rows(), op.cols(), X, k, k, R, k, op.lambda_reg, X, k,
                                         op.clamp_for(LAKER_CLAMP_AUTO), B, st));

@@ -1,0 +1,256 @@
+"""Continuous-batching solve server for streams of LAKER systems on one GPU.
+
+Requests (locations, targets with k right-hand sides, embedding_dim) arrive
+over time.  The server keeps ``slots`` solve lanes, each a host worker bound
+to its own HIP stream; a dispatcher admits a waiting request as soon as a lane
+is free *and* the request's HBM footprint fits the remaining budget, so
+requests that arrive while others are on the GPU start while those are still
+running and their kernels overlap on the device.  Each admitted request runs
+the full device path -- embedding, operator packing, CCCP build, mixed- or
+FP32-precision PCG -- entirely stream-ordered on its lane; E, the factors and
+the Krylov state stay resident until the request completes.
+
+Results are deterministic per request (fixed-order reductions, probes seeded
+by the request id), so a request's solution is bit-identical whether it is
+solved alone or in the mix.  Latency = completion - arrival (queueing
+included); the server reports solves/sec and p50/p99.
+
+No reference counterpart: the reference solves one system per call
+(laker/models.py:392-455).
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import math
+import threading
+import time
+from typing import Callable, List, Optional
+
+import torch
+
+
+@dataclasses.dataclass
+class SolveRequest:
+    rid: int
+    x: torch.Tensor            # (n, 2) locations (host or device)
+    y: torch.Tensor            # (n,) or (n, k) targets
+    embedding_dim: int = 10
+    arrival: float = 0.0       # seconds since the start of the stream
+    lambda_reg: float = 1e-2
+
+    @property
+    def n(self) -> int:
+        return self.x.shape[0]
+
+    @property
+    def k(self) -> int:
+        return 1 if self.y.dim() == 1 else self.y.shape[1]
+
+
+@dataclasses.dataclass
+class SolveResult:
+    rid: int
+    alpha: Optional[torch.Tensor]
+    n: int
+    k: int
+    latency_s: float
+    queue_s: float
+    service_s: float
+    iterations: int = 0
+    outer: int = 0
+    relres: float = float("nan")
+    slot: int = -1
+    error: Optional[str] = None
+
+
+def footprint_bytes(n: int, k: int, d: int, nr: Optional[int] = None) -> int:
+    """Peak device bytes of one request (build peak dominates)."""
+    nr = nr if nr is not None else min(n, max(200, int(2 * math.sqrt(n))))
+    ka32, ka16 = 2 * ((d + 1) // 2), 16 * ((4 * d + 15) // 16)
+    npad = 64 * ((n + 63) // 64)
+    packs = npad * (2 * ka32 * 4 + 2 * ka16 * 2) + n * d * 12
+    build = n * nr * (4 + 4 + 8 + 8) + nr * nr * 8 * 12
+    solve = n * k * (6 * 4 + 3 * 8) + 64 * n * max(k, 4) * 4
+    return int(packs + build + solve + (64 << 20))
+
+
+def make_request(rid: int, seed_base: int = 0, n_min: int = 1000, n_max: int = 65536,
+                 k_max: int = 32, arrival: float = 0.0) -> SolveRequest:
+    """The serving-stream request ``rid``: n log-uniform in [n_min, n_max],
+    d in {10, 16}, k in [1, k_max], inputs as in the n=2000 configuration,
+    everything seeded by the request id."""
+    from .data import pathloss_problem
+
+    g = torch.Generator().manual_seed(seed_base * 1_000_003 + rid)
+    u = torch.rand(3, generator=g, dtype=torch.float64)
+    n = int(round(math.exp(math.log(n_min) + float(u[0]) * (math.log(n_max) - math.log(n_min)))))
+    d = 10 if float(u[1]) < 0.5 else 16
+    k = 1 + int(float(u[2]) * k_max)
+    k = min(k, k_max)
+    x, y = pathloss_problem(n, seed=rid, k=k)
+    return SolveRequest(rid=rid, x=x, y=y, embedding_dim=d, arrival=arrival)
+
+
+def poisson_arrivals(count: int, rate: float, seed: int = 0) -> List[float]:
+    g = torch.Generator().manual_seed(seed)
+    gaps = -torch.log(torch.rand(count, generator=g, dtype=torch.float64)) / rate
+    return torch.cumsum(gaps, 0).tolist()
+
+
+class SolveServer:
+    """Continuous-batching server (see module docstring).
+
+    Args:
+        slots: concurrent solve lanes (HIP streams).
+        hbm_budget_bytes: admission cap on the summed request footprints
+            (default: 70% of the GPU's memory).
+        precision: ``"mixed"`` or ``"fp32"``.
+        tol / inner_tol: PCG tolerances.
+        solve_fn: injectable per-request work ``(req, slot) -> SolveResult``
+            (host-side tests of the scheduler use a stand-in).
+    """
+
+    def __init__(self, slots: int = 4, hbm_budget_bytes: Optional[int] = None, precision: str = "mixed",
+                 tol: float = 1e-6, inner_tol: float = 3e-4, gamma: float = 0.1,
+                 solve_fn: Optional[Callable] = None, footprint_fn: Callable = footprint_bytes):
+        self.slots = int(slots)
+        self.precision = precision
+        self.tol = tol
+        self.inner_tol = inner_tol
+        self.gamma = gamma
+        self._solve_fn = solve_fn
+        self._footprint = footprint_fn
+        if hbm_budget_bytes is None:
+            hbm_budget_bytes = int(0.7 * torch.cuda.get_device_properties(0).total_memory) \
+                if torch.cuda.is_available() else 1 << 40
+        self.budget = int(hbm_budget_bytes)
+        self._emb = {}
+        self._streams = None
+        self.max_inflight = 0
+        self.max_bytes_inflight = 0
+
+    # ------------------------------------------------------------------
+    def _embedding(self, d: int):
+        from .backend import compute_device
+        from .embeddings import PositionEmbedding
+
+        if d not in self._emb:
+            self._emb[d] = PositionEmbedding(2, d, device=compute_device())
+        return self._emb[d]
+
+    def solve_one(self, req: SolveRequest, slot: int) -> SolveResult:
+        """Device path of one request on the current stream."""
+        from .backend import compute_device
+        from .kernels import AttentionKernelOperator
+        from .preconditioner import CCCPPreconditioner
+        from .solvers import PreconditionedConjugateGradient
+
+        dev = compute_device()
+        with torch.no_grad():
+            e = self._embedding(req.embedding_dim)(req.x.to(dev)).contiguous()
+        y = req.y.to(dev, torch.float32).contiguous()
+        op = AttentionKernelOperator(e, lambda_reg=req.lambda_reg,
+                                     precision="mixed" if self.precision == "mixed" else "fp32")
+        pre = CCCPPreconditioner(gamma=self.gamma, verbose=False, probe_source="device",
+                                 probe_precision="bf16" if self.precision == "mixed" else "fp32")
+        pre.build(op.matvec, req.n, seed=req.rid)
+        pcg = PreconditionedConjugateGradient(tol=self.tol, max_iter=1000, verbose=False,
+                                              precision=self.precision, inner_tol=self.inner_tol)
+        a = pcg.solve(op.matvec, pre.apply, y)
+        alpha = pcg.solution_fp64 if pcg.solution_fp64 is not None else a
+        relres = pcg.residual_norm / float(torch.linalg.norm(y.double())) if pcg.residual_norm < math.inf else 0.0
+        op.close()
+        pre.close()
+        return SolveResult(rid=req.rid, alpha=alpha, n=req.n, k=req.k, latency_s=0.0, queue_s=0.0, service_s=0.0,
+                           iterations=pcg.iterations, outer=pcg.outer_iterations, relres=relres, slot=slot)
+
+    # ------------------------------------------------------------------
+    def run(self, requests: List[SolveRequest], realtime: bool = True) -> List[SolveResult]:
+        """Serve ``requests`` (sorted by arrival).  With ``realtime`` arrivals are
+        honoured against the wall clock; otherwise every request is available
+        at t=0 (a saturating stream)."""
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        foot = {r.rid: self._footprint(r.n, r.k, r.embedding_dim) for r in reqs}
+        for r in reqs:
+            if foot[r.rid] > self.budget:
+                raise ValueError(f"request {r.rid} needs {foot[r.rid]} bytes > HBM budget {self.budget}")
+        use_gpu = self._solve_fn is None
+        if use_gpu and self._streams is None:
+            self._streams = [torch.cuda.Stream() for _ in range(self.slots)]
+        lock = threading.Condition()
+        free_slots = list(range(self.slots))
+        inflight_bytes = [0]
+        inflight = [0]
+        results: List[SolveResult] = []
+        t0 = time.perf_counter()
+
+        def worker(req: SolveRequest, slot: int, t_admit: float):
+            try:
+                if use_gpu:
+                    with torch.cuda.stream(self._streams[slot]):
+                        res = self.solve_one(req, slot)
+                        torch.cuda.current_stream().synchronize()
+                else:
+                    res = self._solve_fn(req, slot)
+            except Exception as exc:  # reported per request, the server keeps going
+                res = SolveResult(rid=req.rid, alpha=None, n=req.n, k=req.k, latency_s=0, queue_s=0,
+                                  service_s=0, slot=slot, error=repr(exc))
+            t_done = time.perf_counter() - t0
+            arrival = req.arrival if realtime else 0.0
+            res.latency_s = t_done - arrival
+            res.queue_s = t_admit - arrival
+            res.service_s = t_done - t_admit
+            res.slot = slot
+            with lock:
+                results.append(res)
+                free_slots.append(slot)
+                inflight_bytes[0] -= foot[req.rid]
+                inflight[0] -= 1
+                lock.notify_all()
+
+        threads = []
+        i = 0
+        with lock:
+            while i < len(reqs):
+                now = time.perf_counter() - t0
+                r = reqs[i]
+                if realtime and r.arrival > now:
+                    lock.wait(timeout=r.arrival - now)
+                    continue
+                if free_slots and inflight_bytes[0] + foot[r.rid] <= self.budget:
+                    slot = free_slots.pop(0)
+                    inflight_bytes[0] += foot[r.rid]
+                    inflight[0] += 1
+                    self.max_inflight = max(self.max_inflight, inflight[0])
+                    self.max_bytes_inflight = max(self.max_bytes_inflight, inflight_bytes[0])
+                    th = threading.Thread(target=worker, args=(r, slot, now), daemon=True)
+                    th.start()
+                    threads.append(th)
+                    i += 1
+                else:
+                    lock.wait(timeout=0.05)
+        for th in threads:
+            th.join()
+        self.elapsed_s = time.perf_counter() - t0
+        return sorted(results, key=lambda r: r.rid)
+
+    @staticmethod
+    def summary(results: List[SolveResult], elapsed_s: float) -> dict:
+        lat = sorted(r.latency_s for r in results if r.error is None)
+        if not lat:
+            return {"solves": 0}
+
+        def pct(p):
+            idx = min(len(lat) - 1, max(0, int(math.ceil(p / 100.0 * len(lat))) - 1))
+            return lat[idx]
+
+        return {
+            "solves": len(lat),
+            "errors": sum(r.error is not None for r in results),
+            "solves_per_sec": len(lat) / elapsed_s,
+            "rhs_per_sec": sum(r.k for r in results if r.error is None) / elapsed_s,
+            "p50_ms": 1000 * pct(50),
+            "p99_ms": 1000 * pct(99),
+            "elapsed_s": elapsed_s,
+        }
