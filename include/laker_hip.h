@@ -37,7 +37,7 @@ extern "C" {
 
 /* ---- matvec arithmetic */
 #define LAKER_PREC_FP32 0  /* Gram on FP32 MFMA, exp + accumulation in FP32 */
-#define LAKER_PREC_MIXED 1 /* Gram on BF16 MFMA from a hi/lo split of E (4 products), FP32 accumulate */
+#define LAKER_PREC_MIXED 1 /* Gram on BF16 MFMA from a 3-way bf16 split of E (6 products), FP32 accumulate */
 #define LAKER_PREC_FP64 2  /* Gram, exp and accumulation in FP64 (refinement residual) */
 #define LAKER_PREC_BF16 3  /* BF16 Gram and BF16 G.X products (probe matvec of the CCCP build) */
 

@@ -233,36 +233,75 @@ template <class T>
 __global__ __launch_bounds__(256) void qtx_vec_kernel(const T* __restrict__ Q, const T* __restrict__ x,
                                                       int64_t n, int nr, int64_t rows_per_block,
                                                       double* __restrict__ part) {
+  // threads own columns c = c0 + u*256 + tid; rows are consumed in batches of
+  // RB so that RB*U independent loads are in flight per thread.
+  constexpr int U = 4, RB = 4;
   const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
   const int64_t r1 = (r0 + rows_per_block < n) ? r0 + rows_per_block : n;
-  for (int c0 = 0; c0 < nr; c0 += 256 * 4) {
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int64_t i = r0; i < r1; ++i) {
-      const double xi = (double)x[i];
+  for (int c0 = 0; c0 < nr; c0 += 256 * U) {
+    double acc[U] = {0.0, 0.0, 0.0, 0.0};
+    int64_t i = r0;
+    for (; i + RB <= r1; i += RB) {
+      T qv[RB][U];
+      double xi[RB];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int b = 0; b < RB; ++b) {
+        xi[b] = (double)x[i + b];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int c = c0 + u * 256 + threadIdx.x;
+          qv[b][u] = (c < nr) ? Q[(i + b) * nr + c] : (T)0;
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < RB; ++b)
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc[u] = fma((double)qv[b][u], xi[b], acc[u]);
+    }
+    for (; i < r1; ++i) {
+      const double xv = (double)x[i];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
         const int c = c0 + u * 256 + threadIdx.x;
-        if (c < nr) acc[u] = fma((double)Q[i * nr + c], xi, acc[u]);
+        if (c < nr) acc[u] = fma((double)Q[i * nr + c], xv, acc[u]);
       }
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < U; ++u) {
       const int c = c0 + u * 256 + threadIdx.x;
       if (c < nr) part[(int64_t)blockIdx.x * nr + c] = acc[u];
     }
   }
 }
 
-__global__ void reduce_parts_cols_kernel(const double* __restrict__ part, int nb, int nr,
-                                         double* __restrict__ out) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nr) return;
+// out[c] = sum_b part[b][c]; block (64 columns x 4 row groups), fixed order per group
+__global__ __launch_bounds__(256) void reduce_parts_cols_kernel(const double* __restrict__ part, int nb, int nr,
+                                                                double* __restrict__ out) {
+  __shared__ double sm[4][64];
+  const int cx = threadIdx.x & 63, cy = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + cx;
   double s = 0.0;
-  for (int b = 0; b < nb; ++b) s += part[(int64_t)b * nr + c];
-  out[c] = s;
+  if (c < nr)
+    for (int b = cy; b < nb; b += 4) s += part[(int64_t)b * nr + c];
+  sm[cy][cx] = s;
+  __syncthreads();
+  if (cy == 0 && c < nr) out[c] = (sm[0][cx] + sm[1][cx]) + (sm[2][cx] + sm[3][cx]);
 }
 
-// z[i] = f * x[i] + sum_c Q[i][c] w[c]   (one wave per row)
+// y[r] = sum_c S[r][c] w[c]  (one wave per row of the nr x nr FP64 matrix)
+__global__ __launch_bounds__(256) void gemv64_kernel(const double* __restrict__ S, const double* __restrict__ w,
+                                                     double* __restrict__ y, int nr) {
+  const int lane = threadIdx.x & 63;
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= nr) return;
+  double s = 0.0;
+  for (int c = lane; c < nr; c += 64) s = fma(S[(int64_t)r * nr + c], w[c], s);
+  s = wave_sum(s);
+  if (lane == 0) y[r] = s;
+}
+
+// z[i] = f * x[i] + sum_c Q[i][c] w[c]   (one wave per QW_ROWS rows)
+constexpr int QW_ROWS = 2;
 template <class T>
 __global__ __launch_bounds__(256) void qw_vec_kernel(const T* __restrict__ Q, const double* __restrict__ w,
                                                      const T* __restrict__ x, T* __restrict__ z,
@@ -271,12 +310,22 @@ __global__ __launch_bounds__(256) void qw_vec_kernel(const T* __restrict__ Q, co
   for (int c = threadIdx.x; c < nr; c += blockDim.x) ws_lds[c] = w[c];
   __syncthreads();
   const int lane = threadIdx.x & 63;
-  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (i >= n) return;
-  double s = 0.0;
-  for (int c = lane; c < nr; c += 64) s = fma((double)Q[i * nr + c], ws_lds[c], s);
-  s = wave_sum(s);
-  if (lane == 0) z[i] = (T)(f * (double)x[i] + s);
+  const int64_t i0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * QW_ROWS;
+  if (i0 >= n) return;
+  double s[QW_ROWS];
+#pragma unroll
+  for (int b = 0; b < QW_ROWS; ++b) s[b] = 0.0;
+  for (int c = lane; c < nr; c += 64) {
+    const double wc = ws_lds[c];
+#pragma unroll
+    for (int b = 0; b < QW_ROWS; ++b)
+      if (i0 + b < n) s[b] = fma((double)Q[(i0 + b) * nr + c], wc, s[b]);
+  }
+#pragma unroll
+  for (int b = 0; b < QW_ROWS; ++b) {
+    const double t = wave_sum(s[b]);
+    if (lane == 0 && i0 + b < n) z[i0 + b] = (T)(f * (double)x[i0 + b] + t);
+  }
 }
 
 template <class T>
@@ -300,21 +349,21 @@ static int apply_t(Precond& pc, const T* X, int k, int64_t ldx, T* Z, int64_t ld
   double* W2 = pc.w2.as<double>();
   const T* Q = pc.Q.as<T>();
   if (k == 1 && ldx == 1 && ldz == 1) {
-    const int64_t nb = std::min<int64_t>(1024, ceil_div(n, 64));
+    const int64_t nb = std::min<int64_t>(512, ceil_div(n, 16));
     const int64_t rpb = ceil_div(n, nb);
     const int64_t nb_eff = ceil_div(n, rpb);
     LAKER_TRY(pc.part.ensure((size_t)nb_eff * nr * 8));
     hipLaunchKernelGGL((qtx_vec_kernel<T>), dim3((unsigned)nb_eff), dim3(256), 0, st, Q, X, n, nr, rpb,
                        pc.part.as<double>());
     LAKER_KCHECK("qtx_vec_kernel");
-    hipLaunchKernelGGL(reduce_parts_cols_kernel, dim3((unsigned)ceil_div(nr, 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(reduce_parts_cols_kernel, dim3((unsigned)ceil_div(nr, 64)), dim3(256), 0, st,
                        pc.part.as<double>(), (int)nb_eff, nr, W1);
     LAKER_KCHECK("reduce_parts_cols_kernel");
-    LAKER_HIP_CHECK(hipGetLastError());
-    LAKER_TRY((gemm<double, double, double>(false, false, nr, 1, nr, 1.0, pc.S.as<double>(), nr, W1,
-                                             1, 0.0, W2, 1, nullptr, st)));
-    hipLaunchKernelGGL((qw_vec_kernel<T>), dim3((unsigned)ceil_div(n, 4)), dim3(256), nr * sizeof(double),
-                       st, Q, W2, X, Z, n, nr, pc.inv_sqrt_a);
+    hipLaunchKernelGGL(gemv64_kernel, dim3((unsigned)ceil_div(nr, 4)), dim3(256), 0, st, pc.S.as<double>(), W1,
+                       W2, nr);
+    LAKER_KCHECK("gemv64_kernel");
+    hipLaunchKernelGGL((qw_vec_kernel<T>), dim3((unsigned)ceil_div(n, 4 * QW_ROWS)), dim3(256),
+                       nr * sizeof(double), st, Q, W2, X, Z, n, nr, pc.inv_sqrt_a);
     LAKER_KCHECK("qw_vec_kernel");
     LAKER_HIP_CHECK(hipGetLastError());
     return LAKER_OK;

@@ -29,10 +29,11 @@ namespace laker {
 // Packing.  Row-side (B operand) arrays are pre-scaled by c = s*log2(e).
 //   F32:   ka = round_up(d,2) floats per row; MFMA step s contracts dims
 //          {s, s + ka/2} (lane half h takes dim s + h*ka/2).
-//   BF16x4: ka = round_up(4d,16) bf16 per row: col side [hi|hi|lo|lo],
-//          row side [hi'|lo'|hi'|lo'] so that the K=4d contraction equals
-//          (hi+lo).(hi'+lo') exactly in FP32 accumulation.
-// Rows are padded with zeros to a multiple of 32.
+//   BF16 split: ka = round_up(6d,16) bf16 per row from e = hi + mid + lo:
+//          col side [hi|hi|mid|hi|lo|mid], row side [hi'|mid'|hi'|lo'|hi'|mid'],
+//          so the K=6d contraction is the six leading products of the split
+//          (about 2^-24 relative, FP32 accumulation in the MFMA).
+// Rows are padded with zeros to a multiple of 64.
 // ---------------------------------------------------------------------------
 
 template <class TE>
@@ -59,17 +60,24 @@ __global__ void pack_bf16x4_kernel(const TE* __restrict__ E, int64_t n, int d, i
   int64_t i = idx / ka;
   int p = (int)(idx % ka);
   uint16_t cv = 0, rv = 0;
-  if (i < n && p < 4 * d) {
-    int seg = p / d, dim = p % d;
-    float e = (float)E[i * d + dim];
-    uint16_t hi = f32_to_bf16_bits(e);
-    uint16_t lo = f32_to_bf16_bits(e - bf16_bits_to_f32(hi));
-    float es = e * cscale;
-    uint16_t his = f32_to_bf16_bits(es);
-    uint16_t los = f32_to_bf16_bits(es - bf16_bits_to_f32(his));
-    // col side: hi, hi, lo, lo ; row side: hi', lo', hi', lo'
-    cv = (seg < 2) ? hi : lo;
-    rv = (seg & 1) ? los : his;
+  if (i < n && p < 6 * d) {
+    const int seg = p / d, dim = p % d;
+    // three-way bf16 split e = hi + mid + lo (24 significant bits)
+    const float e = (float)E[i * d + dim];
+    const uint16_t hi = f32_to_bf16_bits(e);
+    const float r1 = e - bf16_bits_to_f32(hi);
+    const uint16_t mid = f32_to_bf16_bits(r1);
+    const uint16_t lo = f32_to_bf16_bits(r1 - bf16_bits_to_f32(mid));
+    const float es = e * cscale;
+    const uint16_t his = f32_to_bf16_bits(es);
+    const float s1 = es - bf16_bits_to_f32(his);
+    const uint16_t mids = f32_to_bf16_bits(s1);
+    const uint16_t los = f32_to_bf16_bits(s1 - bf16_bits_to_f32(mids));
+    // six products: hi.hi' + hi.mid' + mid.hi' + hi.lo' + lo.hi' + mid.mid'
+    const uint16_t col_seg[6] = {hi, hi, mid, hi, lo, mid};
+    const uint16_t row_seg[6] = {his, mids, his, los, his, mids};
+    cv = col_seg[seg];
+    rv = row_seg[seg];
   }
   if (colside) colside[idx] = cv;
   if (rowside) rowside[idx] = rv;
@@ -196,11 +204,15 @@ __global__ __launch_bounds__(256) void mv_vec_kernel(
   const int64_t ntiles = (n_cols + 31) / 32;
   const int64_t t0 = ntiles * js / jsplit, t1 = ntiles * (js + 1) / jsplit;
 
-  float acc[ITW][KC];
+  // two accumulators per (row tile, column) halve the FMA dependency chains
+  float acc[ITW][KC][2];
 #pragma unroll
   for (int it = 0; it < ITW; ++it)
 #pragma unroll
-    for (int c = 0; c < KC; ++c) acc[it][c] = 0.f;
+    for (int c = 0; c < KC; ++c) acc[it][c][0] = acc[it][c][1] = 0.f;
+  // k == 1 with a contiguous, 16-byte aligned x: the lane's 16 x values are
+  // four aligned float4 runs x[J0 + 8g + 4h .. +3]
+  const bool vec4 = (KC == 1) && ldx == 1 && ((reinterpret_cast<uintptr_t>(X) & 15) == 0);
 
   for (int64_t t = t0; t < t1; ++t) {
     const int64_t J0 = t * 32;
@@ -208,11 +220,20 @@ __global__ __launch_bounds__(256) void mv_vec_kernel(
     G::load(A, colpack, J0 + li, ka, h);
     float xv[16][KC];
     if (J0 + 32 <= n_cols) {
+      if (vec4) {
 #pragma unroll
-      for (int v = 0; v < 16; ++v)
+        for (int g = 0; g < 4; ++g) {
+          const floatx4 f = *reinterpret_cast<const floatx4*>(X + J0 + 8 * g + 4 * h);
 #pragma unroll
-        for (int c = 0; c < KC; ++c)
-          xv[v][c] = (c < kcols) ? X[(J0 + jj_of(v, h)) * ldx + c] : 0.f;
+          for (int q = 0; q < 4; ++q) xv[4 * g + q][0] = f[q];
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < 16; ++v)
+#pragma unroll
+          for (int c = 0; c < KC; ++c)
+            xv[v][c] = (c < kcols) ? X[(J0 + jj_of(v, h)) * ldx + c] : 0.f;
+      }
     } else {
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
@@ -221,26 +242,36 @@ __global__ __launch_bounds__(256) void mv_vec_kernel(
         for (int c = 0; c < KC; ++c) xv[v][c] = (j < n_cols && c < kcols) ? X[j * ldx + c] : 0.f;
       }
     }
+    // all Gram tiles first (MFMA), then the exponentials (VALU): the MFMA pipe
+    // works on tile it+1 while the VALU drains tile it
+    floatx16 D[ITW];
+#pragma unroll
+    for (int it = 0; it < ITW; ++it) D[it] = G::gram(A, B[it], ka);
 #pragma unroll
     for (int it = 0; it < ITW; ++it) {
-      floatx16 D = G::gram(A, B[it], ka);
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
-        const float e = kexp<CLAMP, DERIV>(D[v], clampv);
+        const float e = kexp<CLAMP, DERIV>(D[it][v], clampv);
 #pragma unroll
-        for (int c = 0; c < KC; ++c) acc[it][c] = __builtin_fmaf(e, xv[v][c], acc[it][c]);
+        for (int c = 0; c < KC; ++c) acc[it][c][v & 1] = __builtin_fmaf(e, xv[v][c], acc[it][c][v & 1]);
       }
     }
   }
+  float accs[ITW][KC];
+#pragma unroll
+  for (int it = 0; it < ITW; ++it)
+#pragma unroll
+    for (int c = 0; c < KC; ++c) accs[it][c] = acc[it][c][0] + acc[it][c][1];
 #pragma unroll
   for (int it = 0; it < ITW; ++it) {
+    float* accr = accs[it];
 #pragma unroll
-    for (int c = 0; c < KC; ++c) acc[it][c] += __shfl_xor(acc[it][c], 32, 64);
+    for (int c = 0; c < KC; ++c) accr[c] += __shfl_xor(accr[c], 32, 64);
     const int64_t i = I0 + it * 32 + li;
     if (h == 0 && i < n_rows) {
 #pragma unroll
       for (int c = 0; c < KC; ++c)
-        if (c < kcols) part[((int64_t)js * n_rows + i) * KC + c] = acc[it][c];
+        if (c < kcols) part[((int64_t)js * n_rows + i) * KC + c] = accr[c];
     }
   }
 }
@@ -771,7 +802,7 @@ int laker_operator_create(const void* E, int64_t n, int32_t d, int32_t dtype, do
   op->clamp_policy = clamp_policy;
   op->clamp_limit = dtype == LAKER_F32 ? 80.0 : 700.0;
   op->ka32 = (int)round_up(d, 2);
-  op->ka16 = (int)round_up(4 * d, 16);
+  op->ka16 = (int)round_up(6 * d, 16);
   int rc = pack_store(op->self, E, dtype, n, d, op->ka32, op->ka16, scale, true, true, stream);
   if (rc != LAKER_OK) {
     delete op;

@@ -488,8 +488,11 @@ int laker_solve_mixed(laker_operator_t oh, laker_precond_t ph, const double* B, 
     LAKER_HIP_CHECK(hipGetLastError());
     laker_pcg_result ir;
     const double eps = std::sqrt(1.1920928955078125e-07);
-    int rc = pcg_run<float>(op, pc, Rf, k, D, nullptr, inner_tol, inner_max_iter, eps, 0,
-                            inner_precision, 8, st, &ir);
+    // Inner target: enough to bring the true residual to ~0.3 tol, but never
+    // tighter than inner_tol (the BF16-split operator's own accuracy floor).
+    const double tau = std::min(0.5, std::max(inner_tol, 0.3 * tol * bnorm / rnorm));
+    int rc = pcg_run<float>(op, pc, Rf, k, D, nullptr, tau, inner_max_iter, eps, 0,
+                            inner_precision, 16, st, &ir);
     result->inner_iterations += ir.iterations;
     if (rc != LAKER_OK) {
       result->status = rc;

@@ -67,7 +67,7 @@ class SolveResult:
 def footprint_bytes(n: int, k: int, d: int, nr: Optional[int] = None) -> int:
     """Peak device bytes of one request (build peak dominates)."""
     nr = nr if nr is not None else min(n, max(200, int(2 * math.sqrt(n))))
-    ka32, ka16 = 2 * ((d + 1) // 2), 16 * ((4 * d + 15) // 16)
+    ka32, ka16 = 2 * ((d + 1) // 2), 16 * ((6 * d + 15) // 16)
     npad = 64 * ((n + 63) // 64)
     packs = npad * (2 * ka32 * 4 + 2 * ka16 * 2) + n * d * 12
     build = n * nr * (4 + 4 + 8 + 8) + nr * nr * 8 * 12
@@ -112,7 +112,7 @@ class SolveServer:
     """
 
     def __init__(self, slots: int = 4, hbm_budget_bytes: Optional[int] = None, precision: str = "mixed",
-                 tol: float = 1e-6, inner_tol: float = 3e-4, gamma: float = 0.1,
+                 tol: float = 1e-6, inner_tol: float = 1e-5, gamma: float = 0.1,
                  solve_fn: Optional[Callable] = None, footprint_fn: Callable = footprint_bytes):
         self.slots = int(slots)
         self.precision = precision
