@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--d", type=int, default=10)
     ap.add_argument("--lambda-reg", type=float, default=1e-2)
     ap.add_argument("--tol", type=float, default=1e-6)
-    ap.add_argument("--inner-tol", type=float, default=1e-5)
+    ap.add_argument("--inner-tol", type=float, default=1e-2)
     ap.add_argument("--oracle-n", type=int, default=20000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")

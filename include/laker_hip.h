@@ -183,13 +183,19 @@ int laker_pcg_solve(laker_operator_t op, laker_precond_t pc, const void* B, int3
                     int32_t restart_freq, int32_t precision, int32_t check_every,
                     hipStream_t stream, laker_pcg_result* result);
 
-/* Mixed-precision solve: inner PCG with BF16-MFMA Gram tiles (LAKER_PREC_MIXED)
- * and FP32 vectors, outer FP64 iterative refinement on the FP64 residual.
- * B, X: n x k FP64.  Stops when ||B - A X||_F / ||B||_F <= tol in FP64.
- * No reference counterpart (new precision mode behind the same solve call). */
+/* Mixed-precision solve (iterative refinement): each outer step runs PCG
+ * from a fresh start on the FP64 residual B - A X scaled to unit norm, with
+ * operator products on BF16-split MFMA Gram tiles (LAKER_PREC_MIXED, or FP32
+ * MFMA with LAKER_PREC_FP32), FP32 vectors and FP64 scalars, and adds its
+ * correction straight into the FP64 X.  A step stops when its recursive
+ * residual has dropped by inner_tol (or by what the outer target still
+ * needs).  B, X: n x k FP64.  Stops when ||B - A X||_F / ||B||_F <= tol in
+ * FP64, after at most max_outer FP64 residuals or inner_max_iter iterations
+ * in total.  No reference counterpart (new precision mode behind the same
+ * solve call). */
 typedef struct laker_mixed_result {
-  int32_t outer_iterations;
-  int32_t inner_iterations; /* total inner PCG iterations */
+  int32_t outer_iterations; /* refinement steps (FP64 residuals) */
+  int32_t inner_iterations; /* total PCG iterations */
   int32_t status;
   int32_t converged;
   double residual_norm;     /* FP64 true residual ||B - A X||_F */

@@ -274,6 +274,121 @@ __global__ __launch_bounds__(256) void qtx_vec_kernel(const T* __restrict__ Q, c
   }
 }
 
+// Q^T x for an FP32 Q with nr % 4 == 0: part[b][c] = sum_{i in chunk b} Q[i][c] x[i].
+// Wave w of the block takes rows r0+w, r0+w+4, ... (four rows in flight);
+// lane l owns the column quads l, l+64, ... (QPL quads, 16-byte loads).  The
+// four waves' partial sums are combined in LDS in a fixed order.
+template <int QPL>
+__global__ __launch_bounds__(256) void qtx4_kernel(const float* __restrict__ Q, const float* __restrict__ x,
+                                                   int64_t n, int nr, int64_t rows_per_block,
+                                                   double* __restrict__ part) {
+  __shared__ double red[4][QPL * 256];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nq = nr >> 2;
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+  const int64_t r1 = (r0 + rows_per_block < n) ? r0 + rows_per_block : n;
+  double acc[QPL][4];
+#pragma unroll
+  for (int u = 0; u < QPL; ++u) acc[u][0] = acc[u][1] = acc[u][2] = acc[u][3] = 0.0;
+  int64_t i = r0 + w;
+  for (; i + 12 < r1; i += 16) {
+    floatx4 q[4][QPL];
+    float xv[4];
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      xv[rr] = x[i + 4 * rr];
+      const floatx4* row = reinterpret_cast<const floatx4*>(Q + (i + 4 * rr) * nr);
+#pragma unroll
+      for (int u = 0; u < QPL; ++u) {
+        const int qi = lane + 64 * u;
+        q[rr][u] = (qi < nq) ? row[qi] : floatx4{0.f, 0.f, 0.f, 0.f};
+      }
+    }
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr)
+#pragma unroll
+      for (int u = 0; u < QPL; ++u)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[u][j] = fma((double)q[rr][u][j], (double)xv[rr], acc[u][j]);
+  }
+  for (; i < r1; i += 4) {
+    const double xv = (double)x[i];
+    const floatx4* row = reinterpret_cast<const floatx4*>(Q + i * nr);
+#pragma unroll
+    for (int u = 0; u < QPL; ++u) {
+      const int qi = lane + 64 * u;
+      if (qi < nq) {
+        const floatx4 q = row[qi];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[u][j] = fma((double)q[j], xv, acc[u][j]);
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < QPL; ++u)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) red[w][(lane + 64 * u) * 4 + j] = acc[u][j];
+  __syncthreads();
+  for (int c = threadIdx.x; c < nr; c += 256)
+    part[(int64_t)blockIdx.x * nr + c] = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
+}
+
+// out[g][c] = sum_{b in group g} part[b][c] with groups of ceil(nb / gridDim.y)
+// rows; block = 64 columns x 4 row phases, fixed order.
+__global__ __launch_bounds__(256) void reduce_parts_groups_kernel(const double* __restrict__ part, int nb,
+                                                                  int nr, double* __restrict__ out) {
+  __shared__ double sm[4][64];
+  const int cx = threadIdx.x & 63, cy = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + cx;
+  const int gsz = (nb + gridDim.y - 1) / gridDim.y;
+  const int b0 = blockIdx.y * gsz, b1 = min(nb, b0 + gsz);
+  double s = 0.0;
+  if (c < nr)
+    for (int b = b0 + cy; b < b1; b += 4) s += part[(int64_t)b * nr + c];
+  sm[cy][cx] = s;
+  __syncthreads();
+  if (cy == 0 && c < nr) out[(int64_t)blockIdx.y * nr + c] = (sm[0][cx] + sm[1][cx]) + (sm[2][cx] + sm[3][cx]);
+}
+
+// z[i] = f * x[i] + sum_c Q[i][c] w[c] for an FP32 Q with nr % 4 == 0: one
+// wave per QW_ROWS rows, 16-byte loads, w staged in LDS.
+template <int QW>
+__global__ __launch_bounds__(256) void qw4_kernel(const float* __restrict__ Q, const double* __restrict__ w,
+                                                  const float* __restrict__ x, float* __restrict__ z,
+                                                  int64_t n, int nr, double f) {
+  extern __shared__ double ws4_lds[];
+  for (int c = threadIdx.x; c < nr; c += blockDim.x) ws4_lds[c] = w[c];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int64_t i0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * QW;
+  if (i0 >= n) return;
+  const int nq = nr >> 2;
+  double s[QW];
+#pragma unroll
+  for (int b = 0; b < QW; ++b) s[b] = 0.0;
+  for (int qi = lane; qi < nq; qi += 64) {
+    const double w0 = ws4_lds[4 * qi], w1 = ws4_lds[4 * qi + 1], w2 = ws4_lds[4 * qi + 2],
+                 w3 = ws4_lds[4 * qi + 3];
+    floatx4 q[QW];
+#pragma unroll
+    for (int b = 0; b < QW; ++b)
+      q[b] = (i0 + b < n) ? reinterpret_cast<const floatx4*>(Q + (i0 + b) * nr)[qi]
+                          : floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int b = 0; b < QW; ++b) {
+      s[b] = fma((double)q[b][0], w0, s[b]);
+      s[b] = fma((double)q[b][1], w1, s[b]);
+      s[b] = fma((double)q[b][2], w2, s[b]);
+      s[b] = fma((double)q[b][3], w3, s[b]);
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < QW; ++b) {
+    const double t = wave_sum(s[b]);
+    if (lane == 0 && i0 + b < n) z[i0 + b] = (float)(f * (double)x[i0 + b] + t);
+  }
+}
+
 // out[c] = sum_b part[b][c]; block (64 columns x 4 row groups), fixed order per group
 __global__ __launch_bounds__(256) void reduce_parts_cols_kernel(const double* __restrict__ part, int nb, int nr,
                                                                 double* __restrict__ out) {
@@ -348,6 +463,38 @@ static int apply_t(Precond& pc, const T* X, int k, int64_t ldx, T* Z, int64_t ld
   double* W1 = pc.w1.as<double>();
   double* W2 = pc.w2.as<double>();
   const T* Q = pc.Q.as<T>();
+  if constexpr (sizeof(T) == 4) {
+    if (k == 1 && ldx == 1 && ldz == 1 && nr % 4 == 0 && nr <= 1024) {
+      const int64_t nb = std::min<int64_t>(1024, ceil_div(n, 64));
+      const int64_t rpb = ceil_div(n, nb);
+      const int nb_eff = (int)ceil_div(n, rpb);
+      constexpr int RG = 8;  // first-level reduction groups
+      LAKER_TRY(pc.part.ensure(((size_t)nb_eff + RG) * nr * 8));
+      double* part = pc.part.as<double>();
+      double* part2 = part + (size_t)nb_eff * nr;
+      const int qpl = (nr / 4 + 63) / 64;
+      const dim3 qg((unsigned)nb_eff), qb(256);
+      if (qpl == 1) hipLaunchKernelGGL((qtx4_kernel<1>), qg, qb, 0, st, Q, X, n, nr, rpb, part);
+      else if (qpl == 2) hipLaunchKernelGGL((qtx4_kernel<2>), qg, qb, 0, st, Q, X, n, nr, rpb, part);
+      else if (qpl == 3) hipLaunchKernelGGL((qtx4_kernel<3>), qg, qb, 0, st, Q, X, n, nr, rpb, part);
+      else hipLaunchKernelGGL((qtx4_kernel<4>), qg, qb, 0, st, Q, X, n, nr, rpb, part);
+      LAKER_KCHECK("qtx4_kernel");
+      const unsigned cb = (unsigned)ceil_div(nr, 64);
+      hipLaunchKernelGGL(reduce_parts_groups_kernel, dim3(cb, RG), dim3(256), 0, st, (const double*)part,
+                         nb_eff, nr, part2);
+      LAKER_KCHECK("reduce_parts_groups_kernel");
+      hipLaunchKernelGGL(reduce_parts_groups_kernel, dim3(cb, 1), dim3(256), 0, st, (const double*)part2, RG,
+                         nr, W1);
+      LAKER_KCHECK("reduce_parts_groups_kernel");
+      hipLaunchKernelGGL(gemv64_kernel, dim3((unsigned)ceil_div(nr, 4)), dim3(256), 0, st, pc.S.as<double>(),
+                         W1, W2, nr);
+      LAKER_KCHECK("gemv64_kernel");
+      hipLaunchKernelGGL((qw4_kernel<2>), dim3((unsigned)ceil_div(n, 8)), dim3(256), nr * sizeof(double), st,
+                         Q, (const double*)W2, X, Z, n, nr, pc.inv_sqrt_a);
+      LAKER_KCHECK("qw4_kernel");
+      return LAKER_OK;
+    }
+  }
   if (k == 1 && ldx == 1 && ldz == 1) {
     const int64_t nb = std::min<int64_t>(512, ceil_div(n, 16));
     const int64_t rpb = ceil_div(n, nb);

@@ -69,10 +69,13 @@ int pack_store(SideStore& s, const void* E, int dtype, int64_t n, int d, int ka3
 // Y (rows.n x k) = [lam * Xr] + exp(min(s*Erow.Ecol^T, c)) X   (FP32 vectors)
 // deriv: use the backward weights (0 where the clamp is active).
 // accumulate: add into Y instead of overwriting.
+// skip: optional device flag; when it reads non-zero at launch time the k <= 4
+// kernels return at once and Y is left untouched (a finished solve's trailing
+// launches cost nothing).
 int fused_product_f32(Operator& op, int precision, const Side& rows, const Side& cols,
                       const float* X, int k, int64_t ldx, float* Y, int64_t ldy, double lam,
                       const float* Xr, int64_t ldxr, bool clamp, bool deriv, int accumulate,
-                      hipStream_t st);
+                      hipStream_t st, const int* skip = nullptr);
 
 // FP64 product; when B != nullptr writes the residual Y = B - (lam Xr + G X).
 template <class XT>

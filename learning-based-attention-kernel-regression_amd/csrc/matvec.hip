@@ -133,6 +133,14 @@ struct GramOps<LAKER_PREC_FP32> {
       if (s < ka / 2) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.v[s], b.v[s], acc, 0, 0, 0);
     return acc;
   }
+  // all NS steps, no run-time step count (NS == ka / 2)
+  template <int NS>
+  __device__ static inline floatx16 gram_exact(const Operand<NS>& a, const Operand<NS>& b) {
+    floatx16 acc = {0};
+#pragma unroll
+    for (int s = 0; s < NS; ++s) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a.v[s], b.v[s], acc, 0, 0, 0);
+    return acc;
+  }
 };
 
 // BF16 MFMA 32x32x16 on the hi/lo packed operands: chunks = ka/16.
@@ -163,6 +171,14 @@ struct GramOps<LAKER_PREC_MIXED> {
       if (q < ka / 16) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.v[q], b.v[q], acc, 0, 0, 0);
     return acc;
   }
+  // all NS chunks, no run-time chunk count (NS == ka / 16)
+  template <int NS>
+  __device__ static inline floatx16 gram_exact(const Operand<NS>& a, const Operand<NS>& b) {
+    floatx16 acc = {0};
+#pragma unroll
+    for (int q = 0; q < NS; ++q) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.v[q], b.v[q], acc, 0, 0, 0);
+    return acc;
+  }
 };
 
 __device__ inline int jj_of(int v, int h) { return 8 * (v >> 2) + 4 * h + (v & 3); }
@@ -189,8 +205,9 @@ template <int MODE, int NS, int KC, int ITW, bool CLAMP, bool DERIV>
 __global__ __launch_bounds__(256) void mv_vec_kernel(
     const void* __restrict__ colpack, const void* __restrict__ rowpack, int ka, int64_t n_rows,
     int64_t n_cols, const float* __restrict__ X, int64_t ldx, int kcols,
-    float* __restrict__ part, int jsplit, float clampv) {
+    float* __restrict__ part, int jsplit, float clampv, const int* __restrict__ skip) {
   using G = GramOps<MODE>;
+  if (skip && *skip) return;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, h = lane >> 5, li = lane & 31;
   const int js = blockIdx.x % jsplit;
   const int64_t ib = blockIdx.x / jsplit;
@@ -273,6 +290,163 @@ __global__ __launch_bounds__(256) void mv_vec_kernel(
       for (int c = 0; c < KC; ++c)
         if (c < kcols) part[((int64_t)js * n_rows + i) * KC + c] = accr[c];
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k = 1 kernel with the column-side operand and x staged through LDS.
+// A block (4 waves x 2 row tiles = 256 rows) walks its column range in chunks
+// of V1_JT column tiles; chunk c+1 is fetched from global memory into
+// registers while chunk c is consumed from LDS (double buffer, one barrier
+// per chunk), so the E / x reads are shared by the 4 waves and their latency
+// hides behind a chunk of exp work.  LDS rows are padded to spread banks.
+// NS must equal the exact step count (ka/2 for FP32, ka/16 for BF16).
+// ---------------------------------------------------------------------------
+constexpr int V1_JT = 4;  // column tiles per LDS chunk
+
+template <int MODE>
+struct V1Layout {
+  static __host__ __device__ inline int gbytes(int ka) { return ka * (MODE == LAKER_PREC_FP32 ? 4 : 2); }
+  static __host__ __device__ inline int lbytes(int ka) {
+    const int g = gbytes(ka);
+    return MODE == LAKER_PREC_FP32 ? ((g / 4) % 2 == 0 ? g + 4 : g) : g + 16;
+  }
+  static __host__ __device__ inline int chunk_bytes(int ka) { return V1_JT * 32 * (lbytes(ka) + 4); }
+};
+
+template <int MODE, int NS, bool CLAMP>
+__global__ __launch_bounds__(256) void mv_vec1_kernel(
+    const void* __restrict__ colpack, const void* __restrict__ rowpack, int ka, int64_t n_rows,
+    int64_t n_cols, const float* __restrict__ X, int64_t ldx, float* __restrict__ part, int jsplit,
+    float clampv, const int* __restrict__ skip) {
+  using G = GramOps<MODE>;
+  constexpr int ITW = 2;
+  if (skip && *skip) return;  // uniform across the block
+  // 16-byte vectors per thread per chunk: V1_JT*32 rows * row bytes / 16 / 256
+  constexpr int VPT = MODE == LAKER_PREC_FP32 ? (8 * NS + 31) / 32 : NS;
+  extern __shared__ __align__(16) char v1_smem[];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, h = lane >> 5, li = lane & 31;
+  const int js = blockIdx.x % jsplit;
+  const int64_t ib = blockIdx.x / jsplit;
+  const int64_t I0 = (ib * 4 + w) * (32 * ITW);
+  const bool active = I0 < n_rows;
+  // row bytes are fixed by NS (ka = 2*NS floats or 16*NS bf16 values)
+  constexpr int gb = MODE == LAKER_PREC_FP32 ? 8 * NS : 32 * NS;
+  constexpr int lb = MODE == LAKER_PREC_FP32 ? ((gb / 4) % 2 == 0 ? gb + 4 : gb) : gb + 16;
+  constexpr int chunk = V1_JT * 32 * (lb + 4);
+  char* const buf0 = v1_smem;
+  char* const buf1 = v1_smem + chunk;
+
+  typename G::template Operand<NS> B[ITW];
+  if (active) {
+#pragma unroll
+    for (int it = 0; it < ITW; ++it) G::load(B[it], rowpack, I0 + it * 32 + li, ka, h);
+  }
+  const int64_t ntiles = (n_cols + 31) / 32;
+  const int64_t t0 = ntiles * js / jsplit, t1 = ntiles * (js + 1) / jsplit;
+  const char* gsrc = reinterpret_cast<const char*>(colpack);
+
+  // The packed side holds round_up(n, 64) rows, so whole tiles are in bounds;
+  // a chunk's rows start 32*t*gb bytes in, a multiple of 16 bytes.
+  uint4 pre[VPT];
+  float prex = 0.f;
+  auto fetch = [&](int64_t t) {
+    const int nt = (int)((t1 - t) < V1_JT ? (t1 - t) : V1_JT);
+    const int nvec = nt * 2 * gb;  // nt*32 rows * gb bytes / 16
+    (void)ka;
+    const uint4* src = reinterpret_cast<const uint4*>(gsrc + t * 32 * gb);
+#pragma unroll
+    for (int u = 0; u < VPT; ++u) {
+      const int vi = u * 256 + threadIdx.x;
+      pre[u] = (vi < nvec) ? src[vi] : uint4{0u, 0u, 0u, 0u};
+    }
+    const int64_t j = t * 32 + threadIdx.x;
+    prex = (threadIdx.x < nt * 32 && j < n_cols) ? X[j * ldx] : 0.f;
+  };
+  auto stash = [&](char* dst) {
+    if constexpr (MODE == LAKER_PREC_FP32) {
+      // rows of ka words may straddle a vector: place word by word
+      constexpr int wpr = gb / 4, lwpr = lb / 4;
+#pragma unroll
+      for (int u = 0; u < VPT; ++u) {
+        const int wi = (u * 256 + threadIdx.x) * 4;
+        const uint32_t vals[4] = {pre[u].x, pre[u].y, pre[u].z, pre[u].w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int word = wi + q;
+          if (word < V1_JT * 32 * wpr)
+            reinterpret_cast<uint32_t*>(dst)[(word / wpr) * lwpr + word % wpr] = vals[q];
+        }
+      }
+    } else {
+      constexpr int vpr = gb / 16;  // BF16 rows are whole multiples of 32 bytes
+#pragma unroll
+      for (int u = 0; u < VPT; ++u) {
+        const int vi = u * 256 + threadIdx.x;
+        if (vi < V1_JT * 32 * vpr)
+          *reinterpret_cast<uint4*>(dst + (vi / vpr) * lb + (vi % vpr) * 16) = pre[u];
+      }
+    }
+    if (threadIdx.x < V1_JT * 32) reinterpret_cast<float*>(dst + V1_JT * 32 * lb)[threadIdx.x] = prex;
+  };
+
+  float acc[ITW][2];
+#pragma unroll
+  for (int it = 0; it < ITW; ++it) acc[it][0] = acc[it][1] = 0.f;
+
+  // unconditional (an empty range stages zeros): the LDS stores wait for every
+  // outstanding global load, the row operands included, once before the loop
+  fetch(t0);
+  stash(buf0);
+  __syncthreads();
+  bool odd = false;
+  for (int64_t tc = t0; tc < t1; tc += V1_JT) {
+    const bool more = tc + V1_JT < t1;
+    if (more) fetch(tc + V1_JT);
+    const int nt = (int)((t1 - tc) < V1_JT ? (t1 - tc) : V1_JT);
+    if (active) {
+      const char* cur = odd ? buf1 : buf0;
+      const float* xs = reinterpret_cast<const float*>(cur + V1_JT * 32 * lb);
+      for (int tt = 0; tt < nt; ++tt) {
+        typename G::template Operand<NS> A;
+        const char* rowp = cur + (tt * 32 + li) * lb;
+        if constexpr (MODE == LAKER_PREC_FP32) {
+          const float* p = reinterpret_cast<const float*>(rowp) + h * NS;
+#pragma unroll
+          for (int s = 0; s < NS; ++s) A.v[s] = p[s];
+        } else {
+#pragma unroll
+          for (int q = 0; q < NS; ++q)
+            A.v[q] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const shortx8*>(rowp + q * 32 + h * 16));
+        }
+        float xv[16];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const floatx4 f = *reinterpret_cast<const floatx4*>(xs + tt * 32 + 8 * g + 4 * h);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) xv[4 * g + q] = f[q];
+        }
+        floatx16 D[ITW];
+#pragma unroll
+        for (int it = 0; it < ITW; ++it) D[it] = G::template gram_exact<NS>(A, B[it]);
+#pragma unroll
+        for (int it = 0; it < ITW; ++it)
+#pragma unroll
+          for (int v = 0; v < 16; ++v)
+            acc[it][v & 1] = __builtin_fmaf(kexp<CLAMP, false>(D[it][v], clampv), xv[v], acc[it][v & 1]);
+      }
+    }
+    if (more) stash(odd ? buf0 : buf1);
+    __syncthreads();
+    odd = !odd;
+  }
+  if (!active) return;
+#pragma unroll
+  for (int it = 0; it < ITW; ++it) {
+    float s = acc[it][0] + acc[it][1];
+    s += __shfl_xor(s, 32, 64);
+    const int64_t i = I0 + it * 32 + li;
+    if (h == 0 && i < n_rows) part[(int64_t)js * n_rows + i] = s;
   }
 }
 
@@ -452,7 +626,8 @@ template <class PT, class XT, class YT, bool RES>
 __global__ void mv_reduce_kernel(const PT* __restrict__ part, int64_t n_rows, int k, int64_t ldp,
                                  int jsplit, double lam, const XT* __restrict__ Xr, int64_t ldx,
                                  const YT* __restrict__ B, YT* __restrict__ Y, int64_t ldy,
-                                 int accumulate) {
+                                 int accumulate, const int* __restrict__ skip) {
+  if (skip && *skip) return;
   int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n_rows * k) return;
   const int64_t i = idx / k;
@@ -521,12 +696,12 @@ static int pick_jsplit(int64_t n_rows, int64_t n_cols, int rows_per_block) {
 static int launch_vec(int mode, const void* colpack, const void* rowpack, int ka,
                       int64_t n_rows, int64_t n_cols, const float* X, int64_t ldx, int kc,
                       float* part, int jsplit, bool clamp, bool deriv, float clampv,
-                      hipStream_t st) {
+                      const int* skip, hipStream_t st) {
   const int rows_per_block = 4 * 64;  // ITW=2
   dim3 grid((unsigned)(ceil_div(n_rows, rows_per_block) * jsplit)), block(256);
 #define LAKER_VEC_LAUNCH(MODE, NS, KC, CL, DV)                                                  \
   hipLaunchKernelGGL((mv_vec_kernel<MODE, NS, KC, 2, CL, DV>), grid, block, 0, st, colpack, rowpack, \
-                     ka, n_rows, n_cols, X, ldx, kc, part, jsplit, clampv)
+                     ka, n_rows, n_cols, X, ldx, kc, part, jsplit, clampv, skip)
 #define LAKER_VEC_KC(MODE, NS, CL, DV)              \
   do {                                              \
     if (kc == 1) LAKER_VEC_LAUNCH(MODE, NS, 1, CL, DV); \
@@ -554,12 +729,56 @@ static int launch_vec(int mode, const void* colpack, const void* rowpack, int ka
     if (ns <= 2) LAKER_VEC_CL(LAKER_PREC_MIXED, 2);
     else if (ns <= 4) LAKER_VEC_CL(LAKER_PREC_MIXED, 4);
     else if (ns <= 8) LAKER_VEC_CL(LAKER_PREC_MIXED, 8);
+    else if (ns <= 12) LAKER_VEC_CL(LAKER_PREC_MIXED, 12);
     else return fail(LAKER_EINVAL, "embedding_dim > 32 not supported by the mixed matvec");
   }
 #undef LAKER_VEC_CL
 #undef LAKER_VEC_KC
 #undef LAKER_VEC_LAUNCH
   LAKER_HIP_CHECK(hipGetLastError());
+  return LAKER_OK;
+}
+
+// k = 1 product through the LDS-staged kernel.  Returns 1 when the shape has
+// no instance (the caller then uses mv_vec_kernel).
+static int launch_vec1(int mode, const void* colpack, const void* rowpack, int ka, int64_t n_rows,
+                       int64_t n_cols, const float* X, int64_t ldx, float* part, int jsplit,
+                       bool clamp, float clampv, const int* skip, hipStream_t st) {
+  const size_t lds = 2 * (size_t)(mode == LAKER_PREC_FP32 ? V1Layout<LAKER_PREC_FP32>::chunk_bytes(ka)
+                                                          : V1Layout<LAKER_PREC_MIXED>::chunk_bytes(ka));
+  if (lds > 65536) return 1;
+  dim3 grid((unsigned)(ceil_div(n_rows, 256) * jsplit)), block(256);
+#define LAKER_V1(MODE, NS)                                                                        \
+  do {                                                                                            \
+    if (clamp)                                                                                    \
+      hipLaunchKernelGGL((mv_vec1_kernel<MODE, NS, true>), grid, block, lds, st, colpack, rowpack, \
+                         ka, n_rows, n_cols, X, ldx, part, jsplit, clampv, skip);                 \
+    else                                                                                          \
+      hipLaunchKernelGGL((mv_vec1_kernel<MODE, NS, false>), grid, block, lds, st, colpack,        \
+                         rowpack, ka, n_rows, n_cols, X, ldx, part, jsplit, clampv, skip);        \
+  } while (0)
+  if (mode == LAKER_PREC_FP32) {
+    switch (ka / 2) {
+      case 2: LAKER_V1(LAKER_PREC_FP32, 2); break;
+      case 3: LAKER_V1(LAKER_PREC_FP32, 3); break;
+      case 4: LAKER_V1(LAKER_PREC_FP32, 4); break;
+      case 5: LAKER_V1(LAKER_PREC_FP32, 5); break;
+      case 8: LAKER_V1(LAKER_PREC_FP32, 8); break;
+      case 16: LAKER_V1(LAKER_PREC_FP32, 16); break;
+      default: return 1;
+    }
+  } else {
+    switch (ka / 16) {
+      case 1: LAKER_V1(LAKER_PREC_MIXED, 1); break;
+      case 2: LAKER_V1(LAKER_PREC_MIXED, 2); break;
+      case 3: LAKER_V1(LAKER_PREC_MIXED, 3); break;
+      case 4: LAKER_V1(LAKER_PREC_MIXED, 4); break;
+      case 6: LAKER_V1(LAKER_PREC_MIXED, 6); break;
+      default: return 1;
+    }
+  }
+#undef LAKER_V1
+  LAKER_KCHECK("mv_vec1_kernel");
   return LAKER_OK;
 }
 
@@ -604,6 +823,7 @@ static int launch_multi(int mode, int gx, const void* colpack, const void* rowpa
     if (ns <= 2) LAKER_MULTI_GX(LAKER_PREC_MIXED, 2);
     else if (ns <= 4) LAKER_MULTI_GX(LAKER_PREC_MIXED, 4);
     else if (ns <= 8) LAKER_MULTI_GX(LAKER_PREC_MIXED, 8);
+    else if (ns <= 12) LAKER_MULTI_GX(LAKER_PREC_MIXED, 12);
     else return fail(LAKER_EINVAL, "embedding_dim > 32 not supported");
   }
 #undef LAKER_MULTI_GX
@@ -652,7 +872,7 @@ static int launch_f64(const double* Ecol, const double* Erow, int d, int64_t n_r
 int fused_product_f32(Operator& op, int precision, const Side& rows, const Side& cols,
                       const float* X, int k, int64_t ldx, float* Y, int64_t ldy, double lam,
                       const float* Xr, int64_t ldxr, bool clamp, bool deriv, int accumulate,
-                      hipStream_t st) {
+                      hipStream_t st, const int* skip) {
   LAKER_CHECK_ARG(k >= 1, "k must be >= 1");
   const float clampv = (float)(op.clamp_limit * kLog2eD);
   const bool mixed = precision == LAKER_PREC_MIXED || precision == LAKER_PREC_BF16;
@@ -665,14 +885,20 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
   if (k <= 4) {
     LAKER_TRY(op.ensure_part((size_t)jsplit * n_rows * 4 * sizeof(float)));
     float* part = op.part.as<float>();
-    LAKER_TRY(launch_vec(gmode, colpack, rowpack, ka, n_rows, n_cols, X, ldx, k, part, jsplit,
-                         clamp, deriv, clampv, st));
+    int rc = 1;
+    if (k == 1 && !deriv)
+      rc = launch_vec1(gmode, colpack, rowpack, ka, n_rows, n_cols, X, ldx, part, jsplit, clamp,
+                       clampv, skip, st);
+    if (rc < 0) return rc;
+    if (rc == 1)
+      LAKER_TRY(launch_vec(gmode, colpack, rowpack, ka, n_rows, n_cols, X, ldx, k, part, jsplit,
+                           clamp, deriv, clampv, skip, st));
     const int kcp = k == 1 ? 1 : (k == 2 ? 2 : 4);
     int64_t tot = n_rows * k;
     hipLaunchKernelGGL((mv_reduce_kernel<float, float, float, false>),
                        dim3((unsigned)ceil_div(tot, 256)), dim3(256), 0, st, part, n_rows, k,
                        (int64_t)kcp, jsplit, lam, Xr, ldxr, (const float*)nullptr, Y, ldy,
-                       accumulate);
+                       accumulate, skip);
     LAKER_KCHECK("mv_reduce_kernel");
     LAKER_HIP_CHECK(hipGetLastError());
     return LAKER_OK;
@@ -693,7 +919,8 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
   int64_t tot = n_rows * k;
   hipLaunchKernelGGL((mv_reduce_kernel<float, float, float, false>),
                      dim3((unsigned)ceil_div(tot, 256)), dim3(256), 0, st, part, n_rows, k, kp,
-                     jsplit, lam, Xr, ldxr, (const float*)nullptr, Y, ldy, accumulate);
+                     jsplit, lam, Xr, ldxr, (const float*)nullptr, Y, ldy, accumulate,
+                     (const int*)nullptr);
   LAKER_KCHECK("mv_reduce_kernel");
   LAKER_HIP_CHECK(hipGetLastError());
   return LAKER_OK;
@@ -718,12 +945,12 @@ int fused_product_f64(Operator& op, const Side& rows, const Side& cols, const XT
   if (B) {
     hipLaunchKernelGGL((mv_reduce_kernel<double, XT, double, true>),
                        dim3((unsigned)ceil_div(tot, 256)), dim3(256), 0, st, part, n_rows, k, ldp,
-                       jsplit, lam, Xr, ldxr, B, Y, ldy, 0);
+                       jsplit, lam, Xr, ldxr, B, Y, ldy, 0, (const int*)nullptr);
     LAKER_KCHECK("mv_reduce_kernel");
   } else {
     hipLaunchKernelGGL((mv_reduce_kernel<double, XT, double, false>),
                        dim3((unsigned)ceil_div(tot, 256)), dim3(256), 0, st, part, n_rows, k, ldp,
-                       jsplit, lam, Xr, ldxr, (const double*)nullptr, Y, ldy, 0);
+                       jsplit, lam, Xr, ldxr, (const double*)nullptr, Y, ldy, 0, (const int*)nullptr);
     LAKER_KCHECK("mv_reduce_kernel");
   }
   LAKER_HIP_CHECK(hipGetLastError());

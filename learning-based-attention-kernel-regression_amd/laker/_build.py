@@ -25,6 +25,9 @@ COMMON_FLAGS = [
     "-fPIC",
     # keep packed FP32 instructions out of the device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA results in ordinary VGPRs: no accumulation-register copies around
+    # the exp / FMA stage that consumes every Gram tile
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-I", os.path.join(os.path.dirname(ROOT), "include"),
 ]
 

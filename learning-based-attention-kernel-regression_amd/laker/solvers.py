@@ -82,7 +82,7 @@ class PreconditionedConjugateGradient:
     def __init__(self, tol: float = 1e-10, max_iter: Optional[int] = None, verbose: bool = True,
                  restart_freq: Optional[int] = None, breakdown_eps: Optional[float] = None,
                  autograd_safe: bool = False, precision: str = "fp32", check_every: int = 8,
-                 inner_tol: float = 1e-5, max_outer: int = 10):
+                 inner_tol: float = 1e-2, max_outer: int = 20):
         if precision not in ("fp32", "mixed"):
             raise ValueError("precision must be 'fp32' or 'mixed'")
         self.tol = float(tol)
@@ -155,16 +155,22 @@ class PreconditionedConjugateGradient:
         n = rhs.shape[0]
         b = rhs.to(device=compute_device(), dtype=torch.float64).contiguous()
         b2 = b.unsqueeze(1) if b.dim() == 1 else b
+        tol = self.tol
+        bnorm = float(torch.linalg.norm(b2))
         if x0 is not None:
-            # warm start: solve for the correction A d = b - A x0
+            # warm start: solve for the correction A d = b - A x0; the stop stays
+            # relative to ||b|| (reference laker/solvers.py:148-166)
             x0d = x0.to(device=compute_device(), dtype=torch.float64).reshape(b2.shape)
             b2 = op.residual_fp64(b2, x0d)
+            r0 = float(torch.linalg.norm(b2))
+            if r0 > 0:
+                tol = min(0.5, self.tol * bnorm / r0)
         k = b2.shape[1]
         x = torch.empty_like(b2)
         res = N.MixedResult()
         max_iter = self.max_iter if self.max_iter is not None else n
         N.check(N.lib.laker_solve_mixed(op.handle, pc.handle if pc is not None else None, N.ptr(b2), k,
-                                        N.ptr(x), self.tol, self.max_outer, self.inner_tol, int(max_iter),
+                                        N.ptr(x), tol, self.max_outer, self.inner_tol, int(max_iter),
                                         N.PREC_MIXED, N.stream_ptr(), ctypes.byref(res)))
         if x0 is not None:
             x = x + x0d
@@ -173,7 +179,7 @@ class PreconditionedConjugateGradient:
         self.residual_norm = float(res.residual_norm)
         self.converged = bool(res.converged)
         if self.verbose:
-            rel = self.residual_norm / res.rhs_norm if res.rhs_norm else 0.0
+            rel = self.residual_norm / bnorm if bnorm else 0.0
             logger.info("mixed PCG: %d outer / %d inner iterations, fp64 rel_res=%.3e",
                         self.outer_iterations, self.iterations, rel)
         x = x.squeeze(1) if rhs.dim() == 1 else x
