@@ -37,7 +37,9 @@ def parse():
     ap.add_argument("--d", type=int, default=10)
     ap.add_argument("--lambda-reg", type=float, default=1e-2)
     ap.add_argument("--tol", type=float, default=1e-6)
-    ap.add_argument("--inner-tol", type=float, default=1e-2)
+    ap.add_argument("--inner-tol", type=float, default=5e-2)
+    ap.add_argument("--mixed-probes", default="mixed", choices=["bf16", "mixed"],
+                    help="precision of the CCCP probe product in the mixed solve")
     ap.add_argument("--oracle-n", type=int, default=20000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
@@ -86,7 +88,7 @@ def main():
     def one_solve(precision):
         op = laker.AttentionKernelOperator(e, lambda_reg=args.lambda_reg, precision=precision)
         pre = laker.CCCPPreconditioner(gamma=0.1, verbose=False, probe_source="device",
-                                       probe_precision="bf16" if precision == "mixed" else "fp32")
+                                       probe_precision=args.mixed_probes if precision == "mixed" else "fp32")
         pre.build(op.matvec, n, seed=0)
         pcg = laker.PreconditionedConjugateGradient(tol=args.tol, max_iter=1000, verbose=False,
                                                     precision=precision, inner_tol=args.inner_tol,
@@ -193,7 +195,7 @@ def main():
             eo = emb(xo.to(dev)).contiguous()
         opo = laker.AttentionKernelOperator(eo, lambda_reg=args.lambda_reg, precision="mixed")
         preo = laker.CCCPPreconditioner(gamma=0.1, verbose=False, probe_source="device",
-                                        probe_precision="bf16").build(opo.matvec, no, seed=0)
+                                        probe_precision=args.mixed_probes).build(opo.matvec, no, seed=0)
         pm = laker.PreconditionedConjugateGradient(tol=args.tol, max_iter=1000, verbose=False, precision="mixed",
                                                    inner_tol=args.inner_tol)
         pm.solve(opo.matvec, preo.apply, yo.to(dev))

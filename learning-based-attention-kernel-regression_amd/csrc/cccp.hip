@@ -541,7 +541,7 @@ static int read_scalar(const void* dev, void* host, size_t bytes, hipStream_t st
 // One Cholesky-QR pass in FP64: G = V^T V (+ shift I), L = chol(G), V <- V L^{-T}, Racc <- L^T Racc.
 // Returns LAKER_ENUMERIC if the factorisation failed.
 static int cholqr_pass(double* V, int64_t n, int nr, double shift, double* G, double* Linv,
-                       double* Racc, double* tmp, double* Vtmp, int* info, DevBuf* ws,
+                       double* Racc, double* tmp, double* Vtmp, int* info, DevBuf* ws, TriWork& tw,
                        hipStream_t st) {
   LAKER_TRY((gemm<double, double, double>(true, false, nr, nr, n, 1.0, V, nr, V, nr, 0.0, G, nr, ws, st)));
   if (shift > 0.0) {
@@ -550,13 +550,13 @@ static int cholqr_pass(double* V, int64_t n, int nr, double shift, double* G, do
     LAKER_KCHECK("add_diag_kernel");
   }
   LAKER_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int), st));
-  LAKER_TRY(potrf_lower(G, nr, nr, info, ws, st));
+  LAKER_TRY(potrf_lower(G, nr, nr, info, ws, tw, st));
   int hinfo = 0;
   LAKER_TRY(read_scalar(info, &hinfo, sizeof(int), st));
   if (hinfo != 0) return LAKER_ENUMERIC;
   // Linv = L^{-1}
   LAKER_TRY(set_identity(Linv, nr, 1.0, st));
-  LAKER_TRY(trsm_lower_left(G, nr, nr, Linv, nr, nr, ws, st));
+  LAKER_TRY(trsm_lower_left(G, nr, nr, Linv, nr, nr, ws, tw, st));
   // Vtmp = V Linv^T ; copy back
   LAKER_TRY((gemm<double, double, double>(false, true, n, nr, nr, 1.0, V, nr, Linv, nr, 0.0, Vtmp, nr,
                                            nullptr, st)));
@@ -596,6 +596,7 @@ static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, i
 
   // 3. Cholesky-QR2 in FP64 (shifted Cholesky-QR3 on failure)
   DevBuf V, Vtmp, G, Linv, Racc, tmp, infob;
+  TriWork tri;
   LAKER_TRY(V.ensure((size_t)n * nr * 8));
   LAKER_TRY(Vtmp.ensure((size_t)n * nr * 8));
   LAKER_TRY(G.ensure(nn * 8));
@@ -614,21 +615,21 @@ static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, i
   };
   LAKER_TRY(load_v());
   int rc = cholqr_pass(V.as<double>(), n, nr, 0.0, G.as<double>(), Linv.as<double>(), Racc.as<double>(),
-                       tmp.as<double>(), Vtmp.as<double>(), infob.as<int>(), &pc->ws, st);
+                       tmp.as<double>(), Vtmp.as<double>(), infob.as<int>(), &pc->ws, tri, st);
   if (rc == LAKER_ENUMERIC) {
     // shifted Cholesky-QR: shift = 11 (n nr + nr (nr + 1)) u ||U||_F^2, ||U||_F^2 <= nr
     LAKER_TRY(load_v());
     const double u = 1.1102230246251565e-16;
     const double shift = 11.0 * ((double)n * nr + (double)nr * (nr + 1)) * u * (double)nr;
     rc = cholqr_pass(V.as<double>(), n, nr, shift, G.as<double>(), Linv.as<double>(), Racc.as<double>(),
-                     tmp.as<double>(), Vtmp.as<double>(), infob.as<int>(), &pc->ws, st);
+                     tmp.as<double>(), Vtmp.as<double>(), infob.as<int>(), &pc->ws, tri, st);
     if (rc == LAKER_OK)
       rc = cholqr_pass(V.as<double>(), n, nr, 0.0, G.as<double>(), Linv.as<double>(), Racc.as<double>(),
-                       tmp.as<double>(), Vtmp.as<double>(), infob.as<int>(), &pc->ws, st);
+                       tmp.as<double>(), Vtmp.as<double>(), infob.as<int>(), &pc->ws, tri, st);
   }
   if (rc == LAKER_OK)
     rc = cholqr_pass(V.as<double>(), n, nr, 0.0, G.as<double>(), Linv.as<double>(), Racc.as<double>(),
-                     tmp.as<double>(), Vtmp.as<double>(), infob.as<int>(), &pc->ws, st);
+                     tmp.as<double>(), Vtmp.as<double>(), infob.as<int>(), &pc->ws, tri, st);
   if (rc == LAKER_ENUMERIC) return fail(LAKER_ENUMERIC, "CCCP build: Cholesky-QR of the probe responses failed");
   LAKER_TRY(rc);
   U.release();
@@ -676,9 +677,9 @@ static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, i
                        Mb.as<double>(), (int64_t)nr, a);
     LAKER_KCHECK("form_m_kernel");
     LAKER_HIP_CHECK(hipMemsetAsync(infob.ptr, 0, sizeof(int), st));
-    LAKER_TRY(potrf_lower(Mb.as<double>(), nr, nr, infob.as<int>(), &pc->ws, st));
+    LAKER_TRY(potrf_lower(Mb.as<double>(), nr, nr, infob.as<int>(), &pc->ws, tri, st));
     LAKER_HIP_CHECK(hipMemcpyAsync(Zb.ptr, Rq, nn * 8, hipMemcpyDeviceToDevice, st));
-    LAKER_TRY(trsm_lower_left(Mb.as<double>(), nr, nr, Zb.as<double>(), nr, nr, &pc->ws, st));
+    LAKER_TRY(trsm_lower_left(Mb.as<double>(), nr, nr, Zb.as<double>(), nr, nr, &pc->ws, tri, st));
     hipLaunchKernelGGL(cccp_weights_kernel, dim3(nr), dim3(256), 0, st, Zb.as<double>(), Rq, Rw.as<double>(),
                        nr, n_over_nr, epsilon);
     LAKER_KCHECK("cccp_weights_kernel");

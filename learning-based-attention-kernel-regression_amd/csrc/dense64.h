@@ -11,14 +11,23 @@ int gemm(bool tra, bool trb, int64_t M, int64_t N, int64_t K, double alpha, cons
          int64_t lda, const TB* B, int64_t ldb, double beta, TC* C, int64_t ldc, DevBuf* ws,
          hipStream_t st);
 
+// Work space of potrf_lower / trsm_lower_left: the inverses of the 64 x 64
+// diagonal blocks of the last factor (dinv) and a panel buffer.
+struct TriWork {
+  DevBuf dinv;
+  DevBuf panel;
+};
+
 // In-place lower Cholesky of the n x n matrix A (upper triangle zeroed).
 // info_dev (device int, caller-zeroed) receives index+1 of the first
-// non-positive pivot.
-int potrf_lower(double* A, int64_t lda, int64_t n, int* info_dev, DevBuf* ws, hipStream_t st);
+// non-positive pivot.  Leaves the diagonal-block inverses of L in tw.
+int potrf_lower(double* A, int64_t lda, int64_t n, int* info_dev, DevBuf* ws, TriWork& tw,
+                hipStream_t st);
 
-// B <- L^{-1} B  (L lower n x n, B n x m).
+// B <- L^{-1} B  (L lower n x n, B n x m), with the diagonal-block inverses
+// that potrf_lower left in tw for this L.
 int trsm_lower_left(const double* L, int64_t ldl, int64_t n, double* B, int64_t ldb, int64_t m,
-                    DevBuf* ws, hipStream_t st);
+                    DevBuf* ws, TriWork& tw, hipStream_t st);
 
 int set_identity(double* A, int64_t n, double diag, hipStream_t st);
 

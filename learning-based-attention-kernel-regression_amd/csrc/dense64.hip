@@ -152,76 +152,68 @@ LAKER_INST_GEMM(double, float, float)
 #undef LAKER_INST_GEMM
 
 // ---------------------------------------------------------------------------
-// Cholesky (lower), blocked with NB = 64.
+// Cholesky (lower), blocked with NB = 64.  The diagonal block is factored in
+// LDS together with the inverse of its factor; the panel solve and the
+// triangular solves then become GEMMs with those 64x64 inverses.
 // ---------------------------------------------------------------------------
 constexpr int PNB = 64;
 
-// Unblocked Cholesky of the nb x nb diagonal block at (k0,k0), in LDS.
-// Sets *info = global index + 1 of the first non-positive pivot.
+// Unblocked Cholesky of the nb x nb diagonal block at (k0,k0), in LDS, and
+// the inverse of the factor into dinv (64 x 64 row-major, identity-padded
+// beyond nb).  Sets *info = global index + 1 of the first non-positive pivot.
 __global__ __launch_bounds__(256) void potrf_diag_kernel(double* __restrict__ A, int64_t lda,
-                                                         int64_t k0, int nb, int* __restrict__ info) {
+                                                         int64_t k0, int nb, int* __restrict__ info,
+                                                         double* __restrict__ dinv) {
   __shared__ double L[PNB][PNB + 1];
-  for (int e = threadIdx.x; e < nb * nb; e += 256) {
-    const int i = e / nb, j = e % nb;
-    L[i][j] = A[(k0 + i) * lda + k0 + j];
-  }
+  const int tid = threadIdx.x, c = tid & 63, rg = tid >> 6;
+  for (int i = rg; i < PNB; i += 4)
+    L[i][c] = (i < nb && c < nb) ? A[(k0 + i) * lda + k0 + c] : (i == c ? 1.0 : 0.0);
   __syncthreads();
   for (int j = 0; j < nb; ++j) {
-    // L[j][j] = sqrt(A[j][j] - sum_{p<j} L[j][p]^2) -- updates are applied right-looking
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
       double v = L[j][j];
       if (!(v > 0.0)) {
         if (*info == 0) *info = (int)(k0 + j + 1);
-        v = 1e300;  // keep going without NaNs; caller checks info
+        v = 1e300;  // keep going without NaNs; the caller checks info
       }
       L[j][j] = sqrt(v);
     }
     __syncthreads();
-    const double djj = L[j][j];
-    for (int i = j + 1 + threadIdx.x; i < nb; i += 256) L[i][j] /= djj;
+    const double rjj = 1.0 / L[j][j];
+    if (tid > j && tid < nb) L[tid][j] *= rjj;
     __syncthreads();
     // trailing update of the lower triangle within the block
-    const int rem = nb - j - 1;
-    for (int e = threadIdx.x; e < rem * rem; e += 256) {
-      const int i = j + 1 + e / rem, c = j + 1 + e % rem;
-      if (c <= i) L[i][c] -= L[i][j] * L[c][j];
-    }
+    if (c > j && c < nb)
+      for (int i = j + 1 + rg; i < nb; i += 4)
+        if (c <= i) L[i][c] -= L[i][j] * L[c][j];
     __syncthreads();
   }
-  for (int e = threadIdx.x; e < nb * nb; e += 256) {
-    const int i = e / nb, j = e % nb;
-    A[(k0 + i) * lda + k0 + j] = (j <= i) ? L[i][j] : 0.0;
+  for (int i = rg; i < nb; i += 4)
+    if (c < nb) A[(k0 + i) * lda + k0 + c] = (c <= i) ? L[i][c] : 0.0;
+  // column t of L^{-1} by forward substitution (entries above t stay zero)
+  if (tid < PNB) {
+    const int t = tid;
+    double x[PNB];
+#pragma unroll
+    for (int i = 0; i < PNB; ++i) {
+      double v = (i == t) ? 1.0 : 0.0;
+#pragma unroll
+      for (int p = 0; p < i; ++p) v -= L[i][p] * x[p];
+      x[i] = (i < t) ? 0.0 : v / L[i][i];
+    }
+#pragma unroll
+    for (int i = 0; i < PNB; ++i) dinv[i * PNB + t] = x[i];
   }
 }
 
-// Rows below the diagonal block: A[r, k0:k0+nb] <- A[r, k0:k0+nb] L_kk^{-T}
-// (forward substitution per row).  One thread per row.
-__global__ __launch_bounds__(256) void potrf_panel_kernel(double* __restrict__ A, int64_t lda,
-                                                          int64_t n, int64_t k0, int nb) {
-  __shared__ double L[PNB][PNB + 1];
-  for (int e = threadIdx.x; e < nb * nb; e += 256) {
-    const int i = e / nb, j = e % nb;
-    L[i][j] = A[(k0 + i) * lda + k0 + j];
-  }
-  __syncthreads();
-  const int64_t r = k0 + nb + (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (r >= n) return;
-  double x[PNB];
-#pragma unroll
-  for (int j = 0; j < PNB; ++j) x[j] = (j < nb) ? A[r * lda + k0 + j] : 0.0;
-#pragma unroll
-  for (int j = 0; j < PNB; ++j) {
-    if (j < nb) {
-      double v = x[j];
-#pragma unroll
-      for (int p = 0; p < PNB; ++p)
-        if (p < j) v -= x[p] * L[j][p];
-      x[j] = v / L[j][j];
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < PNB; ++j)
-    if (j < nb) A[r * lda + k0 + j] = x[j];
+// dst[r][c] = src[r][c] for r < rows, c < cols
+__global__ void copy2d_kernel(const double* __restrict__ src, int64_t lds_, double* __restrict__ dst,
+                              int64_t ldd, int64_t rows, int cols) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= rows * cols) return;
+  const int64_t r = idx / cols;
+  const int c = (int)(idx % cols);
+  dst[r * ldd + c] = src[r * lds_ + c];
 }
 
 __global__ void zero_upper_kernel(double* __restrict__ A, int64_t lda, int64_t n) {
@@ -231,79 +223,73 @@ __global__ void zero_upper_kernel(double* __restrict__ A, int64_t lda, int64_t n
   if (j > i) A[i * lda + j] = 0.0;
 }
 
-int potrf_lower(double* A, int64_t lda, int64_t n, int* info_dev, DevBuf* ws, hipStream_t st) {
+static int copy2d(const double* src, int64_t lds_, double* dst, int64_t ldd, int64_t rows, int cols,
+                  hipStream_t st) {
+  if (rows <= 0 || cols <= 0) return LAKER_OK;
+  hipLaunchKernelGGL(copy2d_kernel, dim3((unsigned)ceil_div(rows * cols, 256)), dim3(256), 0, st, src,
+                     lds_, dst, ldd, rows, cols);
+  LAKER_KCHECK("copy2d_kernel");
+  return LAKER_OK;
+}
+
+int potrf_lower(double* A, int64_t lda, int64_t n, int* info_dev, DevBuf* ws, TriWork& tw,
+                hipStream_t st) {
+  const int64_t nblk = ceil_div(n, PNB);
+  LAKER_TRY(tw.dinv.ensure((size_t)nblk * PNB * PNB * sizeof(double)));
+  LAKER_TRY(tw.panel.ensure((size_t)std::max<int64_t>(n, 1) * PNB * sizeof(double)));
+  double* dinv = tw.dinv.as<double>();
+  double* panel = tw.panel.as<double>();
   for (int64_t k0 = 0; k0 < n; k0 += PNB) {
     const int nb = (int)std::min<int64_t>(PNB, n - k0);
-    hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, st, A, lda, k0, nb, info_dev);
+    double* dk = dinv + (k0 / PNB) * PNB * PNB;
+    hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, st, A, lda, k0, nb, info_dev, dk);
     LAKER_KCHECK("potrf_diag_kernel");
     const int64_t rem = n - k0 - nb;
     if (rem > 0) {
-      hipLaunchKernelGGL(potrf_panel_kernel, dim3((unsigned)ceil_div(rem, 256)), dim3(256), 0, st, A,
-                         lda, n, k0, nb);
-      LAKER_KCHECK("potrf_panel_kernel");
+      // L21 = A21 L11^{-T} into the panel buffer (rem x nb), then back into A
+      double* A21 = A + (k0 + nb) * lda + k0;
+      LAKER_TRY((gemm<double, double, double>(false, true, rem, nb, nb, 1.0, A21, lda, dk, PNB, 0.0,
+                                               panel, nb, ws, st)));
+      LAKER_TRY(copy2d(panel, nb, A21, lda, rem, nb, st));
       // A22 -= L21 L21^T  (full square of the trailing block; upper part is scratch)
-      double* L21 = A + (k0 + nb) * lda + k0;
       double* A22 = A + (k0 + nb) * lda + k0 + nb;
-      LAKER_TRY((gemm<double, double, double>(false, true, rem, rem, nb, -1.0, L21, lda, L21, lda,
+      LAKER_TRY((gemm<double, double, double>(false, true, rem, rem, nb, -1.0, panel, nb, panel, nb,
                                                1.0, A22, lda, ws, st)));
     }
   }
   hipLaunchKernelGGL(zero_upper_kernel, dim3((unsigned)ceil_div(n * n, 256)), dim3(256), 0, st, A,
                      lda, n);
   LAKER_KCHECK("zero_upper_kernel");
-  LAKER_HIP_CHECK(hipGetLastError());
   return LAKER_OK;
 }
 
 // ---------------------------------------------------------------------------
-// X = L^{-1} B in place (B: n x m, row-major ldb), L lower n x n.
+// X = L^{-1} B in place (B: n x m, row-major ldb), L lower n x n, with the
+// diagonal-block inverses that potrf_lower left in tw.
 // ---------------------------------------------------------------------------
-// Diagonal-block solve: B[k0:k0+nb, c] <- L_kk^{-1} B[k0:k0+nb, c], one thread per column.
-__global__ __launch_bounds__(256) void trsm_diag_kernel(const double* __restrict__ L, int64_t ldl,
-                                                        double* __restrict__ B, int64_t ldb,
-                                                        int64_t m, int64_t k0, int nb) {
-  __shared__ double Ls[PNB][PNB + 1];
-  for (int e = threadIdx.x; e < nb * nb; e += 256) {
-    const int i = e / nb, j = e % nb;
-    Ls[i][j] = L[(k0 + i) * ldl + k0 + j];
-  }
-  __syncthreads();
-  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (c >= m) return;
-  double x[PNB];
-#pragma unroll
-  for (int i = 0; i < PNB; ++i) x[i] = (i < nb) ? B[(k0 + i) * ldb + c] : 0.0;
-#pragma unroll
-  for (int i = 0; i < PNB; ++i) {
-    if (i < nb) {
-      double v = x[i];
-#pragma unroll
-      for (int p = 0; p < PNB; ++p)
-        if (p < i) v -= Ls[i][p] * x[p];
-      x[i] = v / Ls[i][i];
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < PNB; ++i)
-    if (i < nb) B[(k0 + i) * ldb + c] = x[i];
-}
-
 int trsm_lower_left(const double* L, int64_t ldl, int64_t n, double* B, int64_t ldb, int64_t m,
-                    DevBuf* ws, hipStream_t st) {
+                    DevBuf* ws, TriWork& tw, hipStream_t st) {
+  const int64_t nblk = ceil_div(n, PNB);
+  if (tw.dinv.bytes < (size_t)nblk * PNB * PNB * sizeof(double))
+    return fail(LAKER_ESTATE, "trsm_lower_left: no diagonal-block inverses (call potrf_lower first)");
+  LAKER_TRY(tw.panel.ensure((size_t)PNB * std::max<int64_t>(m, 1) * sizeof(double)));
+  const double* dinv = tw.dinv.as<double>();
+  double* panel = tw.panel.as<double>();
   for (int64_t k0 = 0; k0 < n; k0 += PNB) {
     const int nb = (int)std::min<int64_t>(PNB, n - k0);
-    hipLaunchKernelGGL(trsm_diag_kernel, dim3((unsigned)ceil_div(m, 256)), dim3(256), 0, st, L, ldl,
-                       B, ldb, m, k0, nb);
-    LAKER_KCHECK("trsm_diag_kernel");
+    const double* dk = dinv + (k0 / PNB) * PNB * PNB;
+    // B_k <- L_kk^{-1} B_k
+    LAKER_TRY((gemm<double, double, double>(false, false, nb, m, nb, 1.0, dk, PNB, B + k0 * ldb, ldb,
+                                             0.0, panel, m, ws, st)));
+    LAKER_TRY(copy2d(panel, m, B + k0 * ldb, ldb, nb, (int)m, st));
     const int64_t rem = n - k0 - nb;
     if (rem > 0) {
-      // B[k0+nb:, :] -= L[k0+nb:, k0:k0+nb] * B[k0:k0+nb, :]
+      // B[k0+nb:, :] -= L[k0+nb:, k0:k0+nb] * B_k
       LAKER_TRY((gemm<double, double, double>(false, false, rem, m, nb, -1.0,
-                                               L + (k0 + nb) * ldl + k0, ldl, B + k0 * ldb, ldb, 1.0,
+                                               L + (k0 + nb) * ldl + k0, ldl, panel, m, 1.0,
                                                B + (k0 + nb) * ldb, ldb, ws, st)));
     }
   }
-  LAKER_HIP_CHECK(hipGetLastError());
   return LAKER_OK;
 }
 

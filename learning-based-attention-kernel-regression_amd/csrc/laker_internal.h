@@ -39,6 +39,7 @@ struct Operator {
   DevBuf scratch;   // backward right-hand sides
   DevBuf pcg_ws;    // PCG vectors, partial sums and device scalars
   DevBuf mixed_ws;  // refinement residual and FP32 correction vectors
+  DevBuf xpack;     // multi-RHS right-hand sides packed as BF16 MFMA operands
 
   int ensure_part(size_t bytes) { return part.ensure(bytes); }
   bool clamp_for(int policy) const {

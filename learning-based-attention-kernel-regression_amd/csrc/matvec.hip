@@ -564,6 +564,112 @@ __global__ __launch_bounds__(256) void mv_multi_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Multi-RHS kernel with BF16 right-hand sides (GX = LAKER_PREC_BF16: P and X
+// rounded to bf16; GX = LAKER_PREC_MIXED: both split hi/lo, 3 products).
+// X is packed beforehand (pack_x_kernel) straight into the MFMA B-operand
+// layout, so a lane fetches each operand with one 16-byte load; the exp tile
+// P is converted once per row tile and reused for all 64 columns of a pass.
+// Xp index: ((t * 2 + q) * 2 + u) * 64 + lane, 8 bf16 values
+//   X[32 t + jj(8u + v, lane/32)][c_base + 32 q + lane%32], v = 0..7.
+// ---------------------------------------------------------------------------
+__global__ void pack_x_kernel(const float* __restrict__ X, int64_t ldx, int64_t n_cols, int kcols,
+                              int c_base, int64_t ntiles, bf16x8* __restrict__ xh, bf16x8* __restrict__ xl) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ntiles * 256) return;
+  const int lane = (int)(idx & 63), u = (int)((idx >> 6) & 1), q = (int)((idx >> 7) & 1);
+  const int64_t t = idx >> 8;
+  const int h = lane >> 5, li = lane & 31;
+  const int c = c_base + 32 * q + li;
+  bf16x8 hi, lo;
+#pragma unroll
+  for (int v = 0; v < 8; ++v) {
+    const int64_t j = t * 32 + jj_of(8 * u + v, h);
+    const float x = (j < n_cols && c < kcols) ? X[j * ldx + c] : 0.f;
+    const uint16_t hb = f32_to_bf16_bits(x);
+    hi[v] = __builtin_bit_cast(__bf16, hb);
+    lo[v] = __builtin_bit_cast(__bf16, f32_to_bf16_bits(x - bf16_bits_to_f32(hb)));
+  }
+  xh[idx] = hi;
+  if (xl) xl[idx] = lo;
+}
+
+template <int NS, int GX, bool CLAMP>
+__global__ __launch_bounds__(256) void mv_multi2_kernel(
+    const void* __restrict__ colpack, const void* __restrict__ rowpack, int ka, int64_t n_rows,
+    int64_t n_cols, const bf16x8* __restrict__ xh, const bf16x8* __restrict__ xl, int kcols,
+    int c_base, float* __restrict__ part, int64_t ldp, int jsplit, float clampv) {
+  using G = GramOps<LAKER_PREC_MIXED>;
+  constexpr int ITW = 2;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, h = lane >> 5, li = lane & 31;
+  const int js = blockIdx.x % jsplit;
+  const int64_t ib = blockIdx.x / jsplit;
+  const int64_t I0 = (ib * 4 + w) * (32 * ITW);
+  if (I0 >= n_rows) return;
+  (void)ka;
+  G::Operand<NS> B[ITW];
+#pragma unroll
+  for (int it = 0; it < ITW; ++it) G::load(B[it], rowpack, I0 + it * 32 + li, NS * 16, h);
+  const int64_t ntiles = (n_cols + 31) / 32;
+  const int64_t t0 = ntiles * js / jsplit, t1 = ntiles * (js + 1) / jsplit;
+  floatx16 acc[ITW][2];
+#pragma unroll
+  for (int it = 0; it < ITW; ++it) acc[it][0] = acc[it][1] = floatx16{0};
+
+  for (int64_t t = t0; t < t1; ++t) {
+    G::Operand<NS> A;
+    G::load(A, colpack, t * 32 + li, NS * 16, h);
+    bf16x8 Xh[2][2], Xl[2][2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int64_t o = ((t * 2 + q) * 2 + u) * 64 + lane;
+        Xh[q][u] = xh[o];
+        if constexpr (GX == LAKER_PREC_MIXED) Xl[q][u] = xl[o];
+      }
+    floatx16 D[ITW];
+#pragma unroll
+    for (int it = 0; it < ITW; ++it) D[it] = G::gram_exact<NS>(A, B[it]);
+#pragma unroll
+    for (int it = 0; it < ITW; ++it) {
+      // K ordering of chunk u: k = 8h + v  <->  j = jj(8u + v, h)
+      bf16x8 Ph[2], Pl[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 8; ++v) {
+          const float pv = kexp<CLAMP, false>(D[it][8 * u + v], clampv);
+          const uint16_t hb = f32_to_bf16_bits(pv);
+          Ph[u][v] = __builtin_bit_cast(__bf16, hb);
+          if constexpr (GX == LAKER_PREC_MIXED)
+            Pl[u][v] = __builtin_bit_cast(__bf16, f32_to_bf16_bits(pv - bf16_bits_to_f32(hb)));
+        }
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ph[u], Xh[q][u], acc[it][q], 0, 0, 0);
+          if constexpr (GX == LAKER_PREC_MIXED) {
+            acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Pl[u], Xh[q][u], acc[it][q], 0, 0, 0);
+            acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ph[u], Xl[q][u], acc[it][q], 0, 0, 0);
+          }
+        }
+    }
+  }
+  // acc[it][q][r]: row i = I0 + it*32 + jj(r, h) ; column c = c_base + q*32 + li
+#pragma unroll
+  for (int it = 0; it < ITW; ++it)
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t i = I0 + it * 32 + jj_of(r, h);
+        const int c = c_base + q * 32 + li;
+        if (i < n_rows && c < kcols) part[((int64_t)js * n_rows + i) * ldp + c] = acc[it][q][r];
+      }
+}
+
+// ---------------------------------------------------------------------------
 // FP64 kernel: one thread per output row, column rows staged in LDS.
 // grid.x = ceil(n_rows/256) * jsplit.  XT: type of X (float or double).
 // ---------------------------------------------------------------------------
@@ -911,10 +1017,49 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
   const int64_t kp = round_up(k, 32);
   LAKER_TRY(op.ensure_part((size_t)jsplit * n_rows * kp * sizeof(float)));
   float* part = op.part.as<float>();
-  for (int c0 = 0; c0 < k; c0 += 64) {
-    const int ncc = (k - c0) > 32 ? 2 : 1;
-    LAKER_TRY(launch_multi(gmode, gx, colpack, rowpack, ka, n_rows, n_cols, X, ldx, k, c0, ncc,
-                           part, kp, jsplit, clamp, deriv, clampv, st));
+  const int ns16 = op.ka16 / 16;
+  const bool packed_x = mixed && !deriv && gx != LAKER_PREC_FP32 && (ns16 == 2 || ns16 == 4 || ns16 == 6);
+  if (packed_x) {
+    const int64_t ntiles = ceil_div(n_cols, 32);
+    const size_t half = (size_t)ntiles * 256 * sizeof(bf16x8);
+    LAKER_TRY(op.xpack.ensure(2 * half));
+    bf16x8* xh = op.xpack.as<bf16x8>();
+    bf16x8* xl = reinterpret_cast<bf16x8*>(op.xpack.as<char>() + half);
+    const bool split = gx == LAKER_PREC_MIXED;
+    dim3 grid((unsigned)(ceil_div(n_rows, 256) * jsplit)), block(256);
+    for (int c0 = 0; c0 < k; c0 += 64) {
+      hipLaunchKernelGGL(pack_x_kernel, dim3((unsigned)ceil_div(ntiles * 256, 256)), dim3(256), 0, st, X,
+                         ldx, n_cols, k, c0, ntiles, xh, split ? xl : nullptr);
+      LAKER_KCHECK("pack_x_kernel");
+#define LAKER_M2(NS, GXM)                                                                             \
+  do {                                                                                                \
+    if (clamp)                                                                                        \
+      hipLaunchKernelGGL((mv_multi2_kernel<NS, GXM, true>), grid, block, 0, st, colpack, rowpack, ka,   \
+                         n_rows, n_cols, (const bf16x8*)xh, (const bf16x8*)xl, k, c0, part, kp, jsplit, \
+                         clampv);                                                                     \
+    else                                                                                              \
+      hipLaunchKernelGGL((mv_multi2_kernel<NS, GXM, false>), grid, block, 0, st, colpack, rowpack, ka,  \
+                         n_rows, n_cols, (const bf16x8*)xh, (const bf16x8*)xl, k, c0, part, kp, jsplit, \
+                         clampv);                                                                     \
+  } while (0)
+#define LAKER_M2_NS(GXM)                     \
+  do {                                       \
+    if (ns16 == 2) LAKER_M2(2, GXM);         \
+    else if (ns16 == 4) LAKER_M2(4, GXM);    \
+    else LAKER_M2(6, GXM);                   \
+  } while (0)
+      if (split) LAKER_M2_NS(LAKER_PREC_MIXED);
+      else LAKER_M2_NS(LAKER_PREC_BF16);
+#undef LAKER_M2_NS
+#undef LAKER_M2
+      LAKER_KCHECK("mv_multi2_kernel");
+    }
+  } else {
+    for (int c0 = 0; c0 < k; c0 += 64) {
+      const int ncc = (k - c0) > 32 ? 2 : 1;
+      LAKER_TRY(launch_multi(gmode, gx, colpack, rowpack, ka, n_rows, n_cols, X, ldx, k, c0, ncc,
+                             part, kp, jsplit, clamp, deriv, clampv, st));
+    }
   }
   int64_t tot = n_rows * k;
   hipLaunchKernelGGL((mv_reduce_kernel<float, float, float, false>),

@@ -85,7 +85,7 @@ class LAKERCore:
                  preconditioner_strategy: str = "cccp", embedding_dtype: Optional[torch.dtype] = None,
                  device: Optional[Union[str, torch.device]] = None, dtype: Optional[torch.dtype] = None,
                  verbose: bool = True, precision: str = "fp32", probe_source: str = "reference",
-                 probe_precision: str = "fp32", inner_tol: float = 1e-2):
+                 probe_precision: str = "fp32", inner_tol: float = 5e-2):
         self.embedding_dim = embedding_dim
         self.lambda_reg = lambda_reg
         self.gamma = gamma

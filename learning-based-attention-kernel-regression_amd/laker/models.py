@@ -48,7 +48,7 @@ class LAKERRegressor:
                  device: Optional[Union[str, torch.device]] = None, dtype: Optional[torch.dtype] = None,
                  verbose: bool = True, residual_corrector: Optional[nn.Module] = None,
                  precision: str = "fp32", probe_source: str = "reference", probe_precision: str = "fp32",
-                 inner_tol: float = 1e-2):
+                 inner_tol: float = 5e-2):
         # validation (laker/models.py:241-288)
         if embedding_dim <= 0:
             raise ValueError(f"embedding_dim must be positive, got {embedding_dim}")

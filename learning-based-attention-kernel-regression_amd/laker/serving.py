@@ -112,7 +112,7 @@ class SolveServer:
     """
 
     def __init__(self, slots: int = 4, hbm_budget_bytes: Optional[int] = None, precision: str = "mixed",
-                 tol: float = 1e-6, inner_tol: float = 1e-2, gamma: float = 0.1,
+                 tol: float = 1e-6, inner_tol: float = 5e-2, gamma: float = 0.1,
                  solve_fn: Optional[Callable] = None, footprint_fn: Callable = footprint_bytes):
         self.slots = int(slots)
         self.precision = precision

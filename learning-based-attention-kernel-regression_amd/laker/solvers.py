@@ -82,7 +82,7 @@ class PreconditionedConjugateGradient:
     def __init__(self, tol: float = 1e-10, max_iter: Optional[int] = None, verbose: bool = True,
                  restart_freq: Optional[int] = None, breakdown_eps: Optional[float] = None,
                  autograd_safe: bool = False, precision: str = "fp32", check_every: int = 8,
-                 inner_tol: float = 1e-2, max_outer: int = 20):
+                 inner_tol: float = 5e-2, max_outer: int = 20):
         if precision not in ("fp32", "mixed"):
             raise ValueError("precision must be 'fp32' or 'mixed'")
         self.tol = float(tol)
