@@ -317,7 +317,7 @@ struct V1Layout {
 template <int MODE, int NS, bool CLAMP>
 __global__ __launch_bounds__(256) void mv_vec1_kernel(
     const void* __restrict__ colpack, const void* __restrict__ rowpack, int ka, int64_t n_rows,
-    int64_t n_cols, const float* __restrict__ X, int64_t ldx, float* __restrict__ part, int jsplit,
+    int64_t n_cols, const float* __restrict__ X, int64_t ldx, double* __restrict__ part, int jsplit,
     float clampv, const int* __restrict__ skip) {
   using G = GramOps<MODE>;
   constexpr int ITW = 2;
@@ -390,9 +390,11 @@ __global__ __launch_bounds__(256) void mv_vec1_kernel(
     if (threadIdx.x < V1_JT * 32) reinterpret_cast<float*>(dst + V1_JT * 32 * lb)[threadIdx.x] = prex;
   };
 
-  float acc[ITW][2];
+  // per tile: two FP32 chains of 8 products per row; across tiles: FP64
+  // (long FP32 running sums of terms that cancel would dominate the error)
+  double acc64[ITW];
 #pragma unroll
-  for (int it = 0; it < ITW; ++it) acc[it][0] = acc[it][1] = 0.f;
+  for (int it = 0; it < ITW; ++it) acc64[it] = 0.0;
 
   // unconditional (an empty range stages zeros): the LDS stores wait for every
   // outstanding global load, the row operands included, once before the loop
@@ -430,10 +432,16 @@ __global__ __launch_bounds__(256) void mv_vec1_kernel(
 #pragma unroll
         for (int it = 0; it < ITW; ++it) D[it] = G::template gram_exact<NS>(A, B[it]);
 #pragma unroll
-        for (int it = 0; it < ITW; ++it)
+        for (int it = 0; it < ITW; ++it) {
+          float a0 = kexp<CLAMP, false>(D[it][0], clampv) * xv[0];
+          float a1 = kexp<CLAMP, false>(D[it][1], clampv) * xv[1];
 #pragma unroll
-          for (int v = 0; v < 16; ++v)
-            acc[it][v & 1] = __builtin_fmaf(kexp<CLAMP, false>(D[it][v], clampv), xv[v], acc[it][v & 1]);
+          for (int v = 2; v < 16; v += 2) {
+            a0 = __builtin_fmaf(kexp<CLAMP, false>(D[it][v], clampv), xv[v], a0);
+            a1 = __builtin_fmaf(kexp<CLAMP, false>(D[it][v + 1], clampv), xv[v + 1], a1);
+          }
+          acc64[it] += (double)(a0 + a1);
+        }
       }
     }
     if (more) stash(odd ? buf0 : buf1);
@@ -443,7 +451,7 @@ __global__ __launch_bounds__(256) void mv_vec1_kernel(
   if (!active) return;
 #pragma unroll
   for (int it = 0; it < ITW; ++it) {
-    float s = acc[it][0] + acc[it][1];
+    double s = acc64[it];
     s += __shfl_xor(s, 32, 64);
     const int64_t i = I0 + it * 32 + li;
     if (h == 0 && i < n_rows) part[(int64_t)js * n_rows + i] = s;
@@ -726,6 +734,101 @@ __global__ __launch_bounds__(256) void mv_f64_kernel(const double* __restrict__ 
   }
 }
 
+// ---------------------------------------------------------------------------
+// FP64 k = 1 kernel with the Gram tiles on the FP64 MFMA (16x16x4).
+// The row operand is pre-scaled by 256 * s * log2(e), so the MFMA returns
+// t' = 256 t with exp(s g) = 2^t, and the exponential is a table-driven exp2
+// (Tang's method): k = rint(t'), r = t' - k (exact, |r| <= 1/2),
+// 2^t = 2^(k >> 8) * T[k & 255] * 2^(r/256), T[m] = 2^(m/256) in LDS and
+// 2^(r/256) by its Taylor series to degree 5 (truncation < 1e-20).
+// Each wave owns ITW = 4 groups of 16 rows; lane l accumulates row
+// i = I0 + 16 it + l%16 over the columns j = J0 + l/16 + 4 r (r < 4) of each
+// 16-column tile; the four lane groups are summed at the end in fixed order.
+// ---------------------------------------------------------------------------
+__device__ inline double exp2_t256(double tp, const double* __restrict__ tab) {
+  constexpr double c = 0.69314718055994530942 / 256.0;
+  constexpr double p1 = c, p2 = c * c / 2.0, p3 = c * c * c / 6.0, p4 = c * c * c * c / 24.0,
+                   p5 = c * c * c * c * c / 120.0;
+  const double kd = __builtin_rint(tp);
+  const double r = tp - kd;
+  const int ki = (int)kd;
+  double p = __builtin_fma(p5, r, p4);
+  p = __builtin_fma(p, r, p3);
+  p = __builtin_fma(p, r, p2);
+  p = __builtin_fma(p, r, p1);
+  p = __builtin_fma(p, r, 1.0);
+  return __builtin_ldexp(tab[ki & 255] * p, ki >> 8);
+}
+
+template <int STEPS, bool CLAMP, class XT>
+__global__ __launch_bounds__(256) void mv_f64m_kernel(const double* __restrict__ Ecol,
+                                                       const double* __restrict__ Erow, int d,
+                                                       int64_t n_rows, int64_t n_cols, double rscale,
+                                                       double clampv, const XT* __restrict__ X,
+                                                       int64_t ldx, double* __restrict__ part,
+                                                       int jsplit) {
+  constexpr int ITW = 4;
+  __shared__ double tab[256];
+  tab[threadIdx.x] = exp2((double)threadIdx.x * (1.0 / 256.0));
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int m = lane & 15, kq = lane >> 4;
+  const int js = blockIdx.x % jsplit;
+  const int64_t ib = blockIdx.x / jsplit;
+  const int64_t I0 = (ib * 4 + w) * (16 * ITW);
+  if (I0 >= n_rows) return;
+  double b[ITW][STEPS];
+#pragma unroll
+  for (int it = 0; it < ITW; ++it)
+#pragma unroll
+    for (int s = 0; s < STEPS; ++s) {
+      const int64_t i = I0 + 16 * it + m;
+      const int k = kq + 4 * s;
+      b[it][s] = (i < n_rows && k < d) ? Erow[i * d + k] * rscale : 0.0;
+    }
+  const int64_t ntiles = (n_cols + 15) / 16;
+  const int64_t t0 = ntiles * js / jsplit, t1 = ntiles * (js + 1) / jsplit;
+  double acc[ITW];
+#pragma unroll
+  for (int it = 0; it < ITW; ++it) acc[it] = 0.0;
+  for (int64_t t = t0; t < t1; ++t) {
+    const int64_t J0 = t * 16;
+    double a[STEPS];
+#pragma unroll
+    for (int s = 0; s < STEPS; ++s) {
+      const int64_t j = J0 + m;
+      const int k = kq + 4 * s;
+      a[s] = (j < n_cols && k < d) ? Ecol[j * d + k] : 0.0;
+    }
+    double xv[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t j = J0 + kq + 4 * r;
+      xv[r] = (j < n_cols) ? (double)X[j * ldx] : 0.0;
+    }
+#pragma unroll
+    for (int it = 0; it < ITW; ++it) {
+      doublex4 D = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < STEPS; ++s) D = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[it][s], D, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        double tp = D[r];
+        if constexpr (CLAMP) tp = fmin(tp, clampv);
+        acc[it] = __builtin_fma(exp2_t256(tp, tab), xv[r], acc[it]);
+      }
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < ITW; ++it) {
+    double v = acc[it];
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    const int64_t i = I0 + 16 * it + m;
+    if (kq == 0 && i < n_rows) part[(int64_t)js * n_rows + i] = v;
+  }
+}
+
 // Y[i][c] = lam * Xr[i][c] + sum_js part[js][i][c]     (Xr may be null)
 // MODE_RES: Y = B - lam*Xr - sum   (FP64 residual)
 template <class PT, class XT, class YT, bool RES>
@@ -848,7 +951,7 @@ static int launch_vec(int mode, const void* colpack, const void* rowpack, int ka
 // k = 1 product through the LDS-staged kernel.  Returns 1 when the shape has
 // no instance (the caller then uses mv_vec_kernel).
 static int launch_vec1(int mode, const void* colpack, const void* rowpack, int ka, int64_t n_rows,
-                       int64_t n_cols, const float* X, int64_t ldx, float* part, int jsplit,
+                       int64_t n_cols, const float* X, int64_t ldx, double* part, int jsplit,
                        bool clamp, float clampv, const int* skip, hipStream_t st) {
   const size_t lds = 2 * (size_t)(mode == LAKER_PREC_FP32 ? V1Layout<LAKER_PREC_FP32>::chunk_bytes(ka)
                                                           : V1Layout<LAKER_PREC_MIXED>::chunk_bytes(ka));
@@ -993,14 +1096,22 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
     float* part = op.part.as<float>();
     int rc = 1;
     if (k == 1 && !deriv)
-      rc = launch_vec1(gmode, colpack, rowpack, ka, n_rows, n_cols, X, ldx, part, jsplit, clamp,
-                       clampv, skip, st);
+      rc = launch_vec1(gmode, colpack, rowpack, ka, n_rows, n_cols, X, ldx, op.part.as<double>(), jsplit,
+                       clamp, clampv, skip, st);
     if (rc < 0) return rc;
-    if (rc == 1)
-      LAKER_TRY(launch_vec(gmode, colpack, rowpack, ka, n_rows, n_cols, X, ldx, k, part, jsplit,
-                           clamp, deriv, clampv, skip, st));
-    const int kcp = k == 1 ? 1 : (k == 2 ? 2 : 4);
     int64_t tot = n_rows * k;
+    if (rc == 0) {
+      // FP64 partial sums of the LDS-staged k = 1 kernel
+      hipLaunchKernelGGL((mv_reduce_kernel<double, float, float, false>),
+                         dim3((unsigned)ceil_div(tot, 256)), dim3(256), 0, st, op.part.as<double>(),
+                         n_rows, 1, (int64_t)1, jsplit, lam, Xr, ldxr, (const float*)nullptr, Y, ldy,
+                         accumulate, skip);
+      LAKER_KCHECK("mv_reduce_kernel");
+      return LAKER_OK;
+    }
+    LAKER_TRY(launch_vec(gmode, colpack, rowpack, ka, n_rows, n_cols, X, ldx, k, part, jsplit,
+                         clamp, deriv, clampv, skip, st));
+    const int kcp = k == 1 ? 1 : (k == 2 ? 2 : 4);
     hipLaunchKernelGGL((mv_reduce_kernel<float, float, float, false>),
                        dim3((unsigned)ceil_div(tot, 256)), dim3(256), 0, st, part, n_rows, k,
                        (int64_t)kcp, jsplit, lam, Xr, ldxr, (const float*)nullptr, Y, ldy,
@@ -1084,8 +1195,32 @@ int fused_product_f64(Operator& op, const Side& rows, const Side& cols, const XT
   const int64_t ldp = k;
   LAKER_TRY(op.ensure_part((size_t)jsplit * n_rows * k * sizeof(double)));
   double* part = op.part.as<double>();
-  LAKER_TRY(launch_f64<XT>(cols.e64, rows.e64, op.d, n_rows, n_cols, op.scale, clamp,
-                           op.clamp_limit, X, ldx, k, part, ldp, jsplit, st));
+  // t' = 256 s log2(e) g must stay inside int range for the table exp2
+  const double tmax = op.max_sq_norm * std::fabs(op.scale) * kLog2eD * 256.0;
+  if (k == 1 && op.d <= 16 && (clamp || tmax < 1e9)) {
+    const int steps = (op.d + 3) / 4;
+    const double rscale = op.scale * kLog2eD * 256.0;
+    const double clampv = op.clamp_limit * kLog2eD * 256.0;
+    dim3 grid((unsigned)(ceil_div(n_rows, 256) * jsplit)), block(256);
+#define LAKER_F64M(ST, CL)                                                                          \
+  hipLaunchKernelGGL((mv_f64m_kernel<ST, CL, XT>), grid, block, 0, st, cols.e64, rows.e64, op.d, n_rows, \
+                     n_cols, rscale, clampv, X, ldx, part, jsplit)
+#define LAKER_F64M_ST(CL)                        \
+  do {                                           \
+    if (steps == 1) LAKER_F64M(1, CL);           \
+    else if (steps == 2) LAKER_F64M(2, CL);      \
+    else if (steps == 3) LAKER_F64M(3, CL);      \
+    else LAKER_F64M(4, CL);                      \
+  } while (0)
+    if (clamp) LAKER_F64M_ST(true);
+    else LAKER_F64M_ST(false);
+#undef LAKER_F64M_ST
+#undef LAKER_F64M
+    LAKER_KCHECK("mv_f64m_kernel");
+  } else {
+    LAKER_TRY(launch_f64<XT>(cols.e64, rows.e64, op.d, n_rows, n_cols, op.scale, clamp,
+                             op.clamp_limit, X, ldx, k, part, ldp, jsplit, st));
+  }
   int64_t tot = n_rows * k;
   if (B) {
     hipLaunchKernelGGL((mv_reduce_kernel<double, XT, double, true>),
