@@ -40,6 +40,10 @@ struct Operator {
   DevBuf pcg_ws;    // PCG vectors, partial sums and device scalars
   DevBuf mixed_ws;  // refinement residual and FP32 correction vectors
   DevBuf xpack;     // multi-RHS right-hand sides packed as BF16 MFMA operands
+  DevBuf sym_map;   // (row block, segment) of each workgroup of the symmetric product
+  int sym_nwg = 0;
+  bool sym_enabled = true;  // symmetric k = 1 product for the square operator
+  DevBuf part_c;    // column partial sums of the symmetric product
 
   int ensure_part(size_t bytes) { return part.ensure(bytes); }
   bool clamp_for(int policy) const {

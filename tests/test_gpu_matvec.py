@@ -39,7 +39,8 @@ def make(n, d, k, seed, amp=0.6, dtype=np.float32):
 
 
 CASES = [(1, 1, 1), (5, 3, 1), (31, 10, 1), (64, 10, 2), (100, 6, 3), (257, 16, 4), (300, 10, 5),
-         (513, 16, 31), (700, 10, 64), (1000, 32, 70), (2000, 10, 1), (2000, 16, 1)]
+         (513, 16, 31), (700, 10, 64), (1000, 32, 70), (2000, 10, 1), (2000, 16, 1), (5000, 10, 1),
+         (9000, 16, 1)]
 
 
 @pytest.mark.parametrize("n,d,k", CASES)
@@ -110,6 +111,45 @@ def test_cross_matvec(cuda, m, n, k):
     yt_ref = O.cross_matvec_fp64(e, q, xq)
     st = np.exp(e.astype(np.float64) @ q.astype(np.float64).T) @ np.abs(xq.astype(np.float64))
     assert norm_err(yt, yt_ref, st) < 2e-5
+
+
+@pytest.mark.parametrize("n,d", [(300, 10), (2049, 10), (7000, 16)])
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_symmetric_product_matches_general(cuda, n, d, precision):
+    """The square operator's k=1 product (symmetric kernel: each exp once per
+    pair) against the general rectangular kernel on the same embeddings."""
+    e, x = make(n, d, 1, seed=n + 7)
+    print(f"symmetric vs general n={n} d={d} precision={precision}", flush=True)
+    et, xt = torch.from_numpy(e).to(cuda), torch.from_numpy(x).to(cuda)
+    op = laker.AttentionKernelOperator(et, lambda_reg=0.0, precision=precision)
+    y_sym = op.matvec(xt).double().cpu().numpy()
+    y_gen = op.cross_matvec(et, xt).double().cpu().numpy()
+    s = scale_rows(e, 0.0, x)
+    assert norm_err(y_sym, y_gen, s) < 2 * TOL[precision]
+    assert norm_err(y_sym, O.matvec_fp64(e, 0.0, x), s) < TOL[precision]
+    # bit-identical on repeat (fixed-order reductions)
+    assert np.array_equal(op.matvec(xt).double().cpu().numpy(), y_sym)
+
+
+@pytest.mark.parametrize("n,amp", [(3000, 0.6), (1500, 9.0)])
+def test_residual_fp64_single_rhs(cuda, n, amp):
+    """k=1 FP64 residual (FP64 MFMA Gram + table exp2) against NumPy FP64,
+    with and without the clamp."""
+    e, _ = make(n, 10, 1, seed=n, amp=amp)
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(n) * 50
+    b = rng.standard_normal(n)
+    op = laker.AttentionKernelOperator(torch.from_numpy(e).to(cuda), lambda_reg=0.01)
+    r = op.residual_fp64(torch.from_numpy(b).to(cuda), torch.from_numpy(x).to(cuda)).cpu().numpy()
+    e64 = e.astype(np.float64)
+    g = e64 @ e64.T
+    if not op.skip_clamp:
+        g = np.minimum(g, 80.0)
+    kx = np.exp(g)
+    rref = b - (0.01 * x + kx @ x)
+    s = np.abs(b) + 0.01 * np.abs(x) + kx @ np.abs(x)
+    assert (amp > 5) == (not op.skip_clamp)
+    assert np.max(np.abs(r - rref) / s) < 1e-13
 
 
 def test_to_dense_diagonal_kernel_eval(cuda):
