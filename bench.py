@@ -69,6 +69,9 @@ def parse():
     ap.add_argument("--no-oracle", action="store_true")
     ap.add_argument("--matvec-only", type=int, default=0,
                     help="only launch the k=1 matvec this many times per precision (profiling)")
+    ap.add_argument("--serving-requests", type=int, default=24,
+                    help="requests in the bounded serving sample (0 disables it)")
+    ap.add_argument("--serving-slots", type=int, default=4)
     ap.add_argument("--quick", action="store_true")
     return ap.parse_args()
 
@@ -303,6 +306,32 @@ def main():
                           f"{iters} PCG matvecs; {cpu_model}"),
                "matvec_s": t_mv_cpu, "probe_matvec_s": t_probe_cpu}
 
+    # Bounded sample of the serving stream: requests r = 0..R-1 (n log-uniform in
+    # [1000, 65536], d in {10, 16}, 1..32 RHS, all seeded by r), all available at
+    # t=0 (saturating), continuous batching over the slots; against the same
+    # requests solved one after another in plain fp32.
+    serving = None
+    if args.serving_requests > 0:
+        from laker.serving import SolveServer, make_request
+
+        reqs = [make_request(r) for r in range(args.serving_requests)]
+        srv = SolveServer(slots=args.serving_slots, precision="mixed", tol=args.tol, inner_tol=args.inner_tol)
+        srv.run(reqs[:2], realtime=False)  # warm-up (embeddings, code objects)
+        res_s = srv.run(reqs, realtime=False)
+        seq = SolveServer(slots=1, precision="fp32", tol=args.tol)
+        seq.run(reqs[:2], realtime=False)
+        res_q = seq.run(reqs, realtime=False)
+        sm, sq = SolveServer.summary(res_s, srv.elapsed_s), SolveServer.summary(res_q, seq.elapsed_s)
+        worst = max((r.relres for r in res_s if r.error is None), default=float("nan"))
+        serving = {"requests": len(reqs), "slots": args.serving_slots, "arrivals": "all at t=0 (saturating)",
+                   "n_range": [min(r.n for r in reqs), max(r.n for r in reqs)],
+                   "rhs_total": sum(r.k for r in reqs), "max_inflight": srv.max_inflight,
+                   "mixed_continuous": sm, "max_fp64_relres": worst,
+                   "fp32_sequential": sq}
+        print(f"[bench] serving: {sm.get('solves_per_sec', 0):.2f} solves/s p50 {sm.get('p50_ms', 0):.0f} ms "
+              f"p99 {sm.get('p99_ms', 0):.0f} ms; sequential fp32 {sq.get('solves_per_sec', 0):.2f} solves/s",
+              file=sys.stderr, flush=True)
+
     out = {
         "metric": "solves/sec + p50/p99 latency at 1e-6 rel. residual, mixed vs plain fp32 PCG",
         "value": 1000.0 / statistics.mean(lat_m),
@@ -328,6 +357,7 @@ def main():
         "fp64_oracle": oracle,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "serving": serving,
         "host": platform.node(),
     }
     print(json.dumps(out))

@@ -191,8 +191,15 @@ int laker_pcg_solve(laker_operator_t op, laker_precond_t pc, const void* B, int3
  * residual has dropped by inner_tol (or by what the outer target still
  * needs).  B, X: n x k FP64.  Stops when ||B - A X||_F / ||B||_F <= tol in
  * FP64, after at most max_outer FP64 residuals or inner_max_iter iterations
- * in total.  No reference counterpart (new precision mode behind the same
- * solve call). */
+ * in total.  Non-positive curvature inside a step ends that step at its last
+ * good iterate (the reduced-precision operator can be slightly indefinite when
+ * lambda is below its rounding error); only a step that makes no progress at
+ * all returns LAKER_EBREAKDOWN.  The best FP64 iterate is kept: the first
+ * step that does not lower the FP64 residual is undone and followed by one
+ * last step run to the full target (refinement cannot contract when lambda is
+ * below the reduced-precision operator's rounding error); converged = 0 then
+ * reports that the target was not reached.  No reference counterpart (new
+ * precision mode behind the same solve call). */
 typedef struct laker_mixed_result {
   int32_t outer_iterations; /* refinement steps (FP64 residuals) */
   int32_t inner_iterations; /* total PCG iterations */

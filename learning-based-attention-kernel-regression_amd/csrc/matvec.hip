@@ -1018,14 +1018,14 @@ __device__ inline double exp2_t256(double tp, const double* __restrict__ tab) {
   return __builtin_ldexp(tab[ki & 255] * p, ki >> 8);
 }
 
-template <int STEPS, bool CLAMP, class XT>
+template <int STEPS, int KC, int ITW, bool CLAMP, class XT>
 __global__ __launch_bounds__(256) void mv_f64m_kernel(const double* __restrict__ Ecol,
                                                        const double* __restrict__ Erow, int d,
                                                        int64_t n_rows, int64_t n_cols, double rscale,
                                                        double clampv, const XT* __restrict__ X,
-                                                       int64_t ldx, double* __restrict__ part,
+                                                       int64_t ldx, int kcols, int c_base,
+                                                       double* __restrict__ part, int64_t ldp,
                                                        int jsplit) {
-  constexpr int ITW = 4;
   __shared__ double tab[256];
   tab[threadIdx.x] = exp2((double)threadIdx.x * (1.0 / 256.0));
   __syncthreads();
@@ -1046,9 +1046,11 @@ __global__ __launch_bounds__(256) void mv_f64m_kernel(const double* __restrict__
     }
   const int64_t ntiles = (n_cols + 15) / 16;
   const int64_t t0 = ntiles * js / jsplit, t1 = ntiles * (js + 1) / jsplit;
-  double acc[ITW];
+  double acc[ITW][KC];
 #pragma unroll
-  for (int it = 0; it < ITW; ++it) acc[it] = 0.0;
+  for (int it = 0; it < ITW; ++it)
+#pragma unroll
+    for (int c = 0; c < KC; ++c) acc[it][c] = 0.0;
   for (int64_t t = t0; t < t1; ++t) {
     const int64_t J0 = t * 16;
     double a[STEPS];
@@ -1058,11 +1060,13 @@ __global__ __launch_bounds__(256) void mv_f64m_kernel(const double* __restrict__
       const int k = kq + 4 * s;
       a[s] = (j < n_cols && k < d) ? Ecol[j * d + k] : 0.0;
     }
-    double xv[4];
+    double xv[4][KC];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int64_t j = J0 + kq + 4 * r;
-      xv[r] = (j < n_cols) ? (double)X[j * ldx] : 0.0;
+#pragma unroll
+      for (int c = 0; c < KC; ++c)
+        xv[r][c] = (j < n_cols && c_base + c < kcols) ? (double)X[j * ldx + c_base + c] : 0.0;
     }
 #pragma unroll
     for (int it = 0; it < ITW; ++it) {
@@ -1073,17 +1077,22 @@ __global__ __launch_bounds__(256) void mv_f64m_kernel(const double* __restrict__
       for (int r = 0; r < 4; ++r) {
         double tp = D[r];
         if constexpr (CLAMP) tp = fmin(tp, clampv);
-        acc[it] = __builtin_fma(exp2_t256(tp, tab), xv[r], acc[it]);
+        const double ev = exp2_t256(tp, tab);
+#pragma unroll
+        for (int c = 0; c < KC; ++c) acc[it][c] = __builtin_fma(ev, xv[r][c], acc[it][c]);
       }
     }
   }
 #pragma unroll
   for (int it = 0; it < ITW; ++it) {
-    double v = acc[it];
-    v += __shfl_xor(v, 16, 64);
-    v += __shfl_xor(v, 32, 64);
     const int64_t i = I0 + 16 * it + m;
-    if (kq == 0 && i < n_rows) part[(int64_t)js * n_rows + i] = v;
+#pragma unroll
+    for (int c = 0; c < KC; ++c) {
+      double v = acc[it][c];
+      v += __shfl_xor(v, 16, 64);
+      v += __shfl_xor(v, 32, 64);
+      if (kq == 0 && i < n_rows && c_base + c < kcols) part[((int64_t)js * n_rows + i) * ldp + c_base + c] = v;
+    }
   }
 }
 
@@ -1521,26 +1530,39 @@ int fused_product_f64(Operator& op, const Side& rows, const Side& cols, const XT
   double* part = op.part.as<double>();
   // t' = 256 s log2(e) g must stay inside int range for the table exp2
   const double tmax = op.max_sq_norm * std::fabs(op.scale) * kLog2eD * 256.0;
-  if (k == 1 && op.d <= 16 && (clamp || tmax < 1e9)) {
+  if (op.d <= 16 && (clamp || tmax < 1e9)) {
     const int steps = (op.d + 3) / 4;
     const double rscale = op.scale * kLog2eD * 256.0;
     const double clampv = op.clamp_limit * kLog2eD * 256.0;
-    dim3 grid((unsigned)(ceil_div(n_rows, 256) * jsplit)), block(256);
-#define LAKER_F64M(ST, CL)                                                                          \
-  hipLaunchKernelGGL((mv_f64m_kernel<ST, CL, XT>), grid, block, 0, st, cols.e64, rows.e64, op.d, n_rows, \
-                     n_cols, rscale, clampv, X, ldx, part, jsplit)
+    // k = 1: 4 row groups of 16 per wave; k > 1: 2 row groups, up to 8 columns per pass
+    const int itw = k == 1 ? 4 : 2;
+    dim3 grid((unsigned)(ceil_div(n_rows, 64 * itw) * jsplit)), block(256);
+    for (int c0 = 0; c0 < k; c0 += 8) {
+      const int kc = k - c0;
+#define LAKER_F64M(ST, KCT, ITWT, CL)                                                                     \
+  hipLaunchKernelGGL((mv_f64m_kernel<ST, KCT, ITWT, CL, XT>), grid, block, 0, st, cols.e64, rows.e64, op.d, \
+                     n_rows, n_cols, rscale, clampv, X, ldx, k, c0, part, ldp, jsplit)
+#define LAKER_F64M_KC(ST, CL)                                       \
+  do {                                                              \
+    if (k == 1) LAKER_F64M(ST, 1, 4, CL);                           \
+    else if (kc <= 2) LAKER_F64M(ST, 2, 2, CL);                     \
+    else if (kc <= 4) LAKER_F64M(ST, 4, 2, CL);                     \
+    else LAKER_F64M(ST, 8, 2, CL);                                  \
+  } while (0)
 #define LAKER_F64M_ST(CL)                        \
   do {                                           \
-    if (steps == 1) LAKER_F64M(1, CL);           \
-    else if (steps == 2) LAKER_F64M(2, CL);      \
-    else if (steps == 3) LAKER_F64M(3, CL);      \
-    else LAKER_F64M(4, CL);                      \
+    if (steps == 1) LAKER_F64M_KC(1, CL);        \
+    else if (steps == 2) LAKER_F64M_KC(2, CL);   \
+    else if (steps == 3) LAKER_F64M_KC(3, CL);   \
+    else LAKER_F64M_KC(4, CL);                   \
   } while (0)
-    if (clamp) LAKER_F64M_ST(true);
-    else LAKER_F64M_ST(false);
+      if (clamp) LAKER_F64M_ST(true);
+      else LAKER_F64M_ST(false);
 #undef LAKER_F64M_ST
+#undef LAKER_F64M_KC
 #undef LAKER_F64M
-    LAKER_KCHECK("mv_f64m_kernel");
+      LAKER_KCHECK("mv_f64m_kernel");
+    }
   } else {
     LAKER_TRY(launch_f64<XT>(cols.e64, rows.e64, op.d, n_rows, n_cols, op.scale, clamp,
                              op.clamp_limit, X, ldx, k, part, ldp, jsplit, st));

@@ -105,7 +105,7 @@ __device__ inline double wave_sum_part(const double* part, int q, int nblk, int 
 // alpha = rz / pAp with the breakdown test.  One block, one wave per column.
 __global__ void pcg_alpha_kernel(const double* __restrict__ part, int nblk, int k,
                                  const double* __restrict__ rz, double* __restrict__ alpha,
-                                 PcgState* __restrict__ st) {
+                                 PcgState* __restrict__ st, int it) {
   if (st->done) return;
   __shared__ int brk;
   if (threadIdx.x == 0) brk = 0;
@@ -121,7 +121,10 @@ __global__ void pcg_alpha_kernel(const double* __restrict__ part, int nblk, int 
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0 && brk) st->done = PCG_BREAKDOWN;
+  if (threadIdx.x == 0 && brk) {
+    st->done = PCG_BREAKDOWN;
+    st->iterations = it;  // iterations completed before the breakdown
+  }
 }
 
 // x += alpha p ; r -= alpha Ap
@@ -357,7 +360,7 @@ static int pcg_run(Operator& op, Precond* pc, const T* B, int k, T* X, const T* 
   for (int it = 0; it < max_iter; ++it) {
     LAKER_TRY(op_apply<T>(op, precision, b.p, k, b.ap, &b.st->done, st));
     LAKER_TRY(coldots<T>(b, 3, b.p, b.ap, b.p, b.p, b.ap, b.ap, n, k, true, st));
-    hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.alpha, b.st);
+    hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.alpha, b.st, it);
     LAKER_KCHECK("pcg_alpha_kernel");
     hipLaunchKernelGGL((pcg_update_xr_kernel<T, T>), eg, eb, 0, st, b.x, b.r, b.p, b.ap, b.alpha, n, k,
                        b.st);
@@ -487,8 +490,11 @@ int laker_solve_mixed(laker_operator_t oh, laker_precond_t ph, const double* B, 
                   "inner precision must be LAKER_PREC_MIXED or LAKER_PREC_FP32");
   Precond* pc = reinterpret_cast<Precond*>(ph);
   const int64_t n = op.n, nk = n * k;
-  LAKER_TRY(op.mixed_ws.ensure(round_up(nk * 8, 256)));
+  const size_t vb = round_up(nk * 8, 256);
+  LAKER_TRY(op.mixed_ws.ensure(3 * vb));
   double* R = op.mixed_ws.as<double>();
+  double* Xbest = reinterpret_cast<double*>(op.mixed_ws.as<char>() + vb);
+  double* Rbest = reinterpret_cast<double*>(op.mixed_ws.as<char>() + 2 * vb);
   PcgBuffers<float> b;
   LAKER_TRY(pcg_buffers<float>(op, k, b));
   const int nblk = b.nblk;
@@ -517,13 +523,19 @@ int laker_solve_mixed(laker_operator_t oh, laker_precond_t ph, const double* B, 
   // residual, so a step stops once its recursive residual has dropped by
   // inner_tol (or by what the outer target still needs); the next FP64
   // residual then starts the next step.
+  // Safeguard: when lambda is below the reduced-precision operator's error
+  // (u ||K|| > lambda), refinement stops contracting.  The best FP64 iterate
+  // is kept; the first step that fails to improve it is undone and replaced by
+  // one last step run to the full target from the best point.
   const double* Rin = B;
-  double rnorm = bnorm;
+  double rnorm = bnorm, rbest = bnorm;
+  bool have_best = false, final_step = false;
   int it = 0;
   const int check_every = 8;
   PcgState hs;
   while (true) {
-    const double seg_tol = std::min(0.5, std::max(inner_tol, 0.3 * tol * bnorm / rnorm));
+    const double need = 0.3 * tol * bnorm / rnorm;
+    const double seg_tol = std::min(0.5, final_step ? need : std::max(inner_tol, need));
     hipLaunchKernelGGL(scale_to_f32_kernel, eg, eb, 0, st, Rin, b.r, nk, 1.0 / rnorm);
     LAKER_KCHECK("scale_to_f32_kernel");
     if (pc) LAKER_TRY(precond_apply(*pc, b.r, k, k, b.z, k, st));
@@ -534,10 +546,11 @@ int laker_solve_mixed(laker_operator_t oh, laker_precond_t ph, const double* B, 
     hipLaunchKernelGGL(pcg_setup_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.st, seg_tol,
                        eps);
     LAKER_KCHECK("pcg_setup_kernel");
+    const int seg_start = it;
     while (it < inner_max_iter) {
       LAKER_TRY(op_apply<float>(op, inner_precision, b.p, k, b.ap, &b.st->done, st));
       LAKER_TRY(coldots<float>(b, 3, b.p, b.ap, b.p, b.p, b.ap, b.ap, n, k, true, st));
-      hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.alpha, b.st);
+      hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.alpha, b.st, it);
       LAKER_KCHECK("pcg_alpha_kernel");
       hipLaunchKernelGGL(pcg_update_x64_kernel, eg, eb, 0, st, X, b.r, (const float*)b.p,
                          (const float*)b.ap, (const double*)b.alpha, rnorm, n, k, (const PcgState*)b.st);
@@ -564,27 +577,50 @@ int laker_solve_mixed(laker_operator_t oh, laker_precond_t ph, const double* B, 
     LAKER_HIP_CHECK(hipStreamSynchronize(st));
     if (hs.done != PCG_RUNNING) it = hs.iterations;  // launches after the stop were no-ops
     result->inner_iterations = it;
-    if (hs.done == PCG_BREAKDOWN) {
+    if (hs.done == PCG_BREAKDOWN && it == seg_start) {
+      // no progress at all in this step: the operator is indefinite here
       result->status = LAKER_EBREAKDOWN;
       return fail(LAKER_EBREAKDOWN,
                   "PCG breakdown: non-positive curvature detected (p^T A p <= 0). The operator may "
                   "be indefinite or the preconditioner may be unsuitable.");
     }
+    // A breakdown after some progress ends the step at the last good iterate:
+    // the reduced-precision operator can be slightly indefinite (rounding of
+    // order u ||K|| against lambda), the FP64 residual below is exact.
     // FP64 residual R = B - A X
     LAKER_TRY(fused_product_f64<double>(op, op.rows(), op.cols(), X, k, k, R, k, op.lambda_reg, X, k,
                                         clamp, B, st));
     LAKER_TRY(frob_norm(R, nk, tmp, &rnorm, st));
     result->outer_iterations += 1;
-    result->residual_norm = rnorm;
     if (rnorm / bnorm <= tol) {
       result->converged = 1;
+      result->residual_norm = rnorm;
       break;
     }
-    if (result->outer_iterations >= max_outer || it >= inner_max_iter || !std::isfinite(rnorm) ||
-        rnorm == 0.0)
-      break;
+    if (std::isfinite(rnorm) && rnorm < rbest) {
+      rbest = rnorm;
+      have_best = true;
+      LAKER_HIP_CHECK(hipMemcpyAsync(Xbest, X, nk * 8, hipMemcpyDeviceToDevice, st));
+      LAKER_HIP_CHECK(hipMemcpyAsync(Rbest, R, nk * 8, hipMemcpyDeviceToDevice, st));
+      if (final_step) break;
+    } else {
+      // no improvement: back to the best iterate (or to zero)
+      if (have_best) {
+        LAKER_HIP_CHECK(hipMemcpyAsync(X, Xbest, nk * 8, hipMemcpyDeviceToDevice, st));
+        LAKER_HIP_CHECK(hipMemcpyAsync(R, Rbest, nk * 8, hipMemcpyDeviceToDevice, st));
+      } else {
+        LAKER_HIP_CHECK(hipMemsetAsync(X, 0, nk * 8, st));
+        LAKER_HIP_CHECK(hipMemcpyAsync(R, B, nk * 8, hipMemcpyDeviceToDevice, st));
+      }
+      rnorm = rbest;
+      if (final_step) break;
+      final_step = true;
+    }
+    result->residual_norm = rnorm;
+    if (result->outer_iterations >= max_outer || it >= inner_max_iter || rnorm == 0.0) break;
     Rin = R;
   }
+  result->residual_norm = std::min(rnorm, rbest);
   result->inner_iterations = it;
   return LAKER_OK;
 }

@@ -252,6 +252,10 @@ class LAKERCore:
                 k_xq = kernel_operator.kernel_eval(embeddings, qc).to(compute_device())
                 v = pcg.solve(kernel_operator.matvec, preconditioner.apply, k_xq)
                 k_qq = kernel_operator.kernel_eval(qc, qc).to(compute_device()).diagonal()
-                var[s:e] = (k_qq.double() - _coldot(k_xq, v.to(k_xq.dtype))).to(self.dtype)
+                if getattr(pcg, "solution_fp64", None) is not None and self.precision == "mixed":
+                    # the refined solution stays FP64: var is a small difference of large terms
+                    var[s:e] = (k_qq.double() - _coldot(k_xq.double(), pcg.solution_fp64)).to(self.dtype)
+                else:
+                    var[s:e] = (k_qq.double() - _coldot(k_xq, v.to(k_xq.dtype))).to(self.dtype)
             self.last_solver = pcg
             return var.clamp(min=0.0).to(self.device)

@@ -72,7 +72,12 @@ def footprint_bytes(n: int, k: int, d: int, nr: Optional[int] = None) -> int:
     packs = npad * (2 * ka32 * 4 + 2 * ka16 * 2) + n * d * 12
     build = n * nr * (4 + 4 + 8 + 8) + nr * nr * 8 * 12
     solve = n * k * (6 * 4 + 3 * 8) + 64 * n * max(k, 4) * 4
-    return int(packs + build + solve + (64 << 20))
+    # partial sums of the symmetric k=1 product (column partials ~n^2/512
+    # doubles, row partials per 2048-column segment) and the packed BF16
+    # right-hand sides of the multi-RHS product
+    sym = (n * n // 512 + (n // 2048 + 1) * n) * 8
+    xpack = ((n + 31) // 32) * 256 * 16 * 2
+    return int(packs + build + solve + sym + xpack + (64 << 20))
 
 
 def make_request(rid: int, seed_base: int = 0, n_min: int = 1000, n_max: int = 65536,
@@ -153,7 +158,7 @@ class SolveServer:
         op = AttentionKernelOperator(e, lambda_reg=req.lambda_reg,
                                      precision="mixed" if self.precision == "mixed" else "fp32")
         pre = CCCPPreconditioner(gamma=self.gamma, verbose=False, probe_source="device",
-                                 probe_precision="bf16" if self.precision == "mixed" else "fp32")
+                                 probe_precision="mixed" if self.precision == "mixed" else "fp32")
         pre.build(op.matvec, req.n, seed=req.rid)
         pcg = PreconditionedConjugateGradient(tol=self.tol, max_iter=1000, verbose=False,
                                               precision=self.precision, inner_tol=self.inner_tol)
