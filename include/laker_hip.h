@@ -36,10 +36,13 @@ extern "C" {
 #define LAKER_F64 1
 
 /* ---- matvec arithmetic */
-#define LAKER_PREC_FP32 0  /* Gram on FP32 MFMA, exp + accumulation in FP32 */
-#define LAKER_PREC_MIXED 1 /* Gram on BF16 MFMA from a 3-way bf16 split of E (6 products), FP32 accumulate */
+#define LAKER_PREC_FP32 0  /* Gram on FP32 MFMA, FP32 exp and products */
+#define LAKER_PREC_MIXED 1 /* Gram on BF16 MFMA from a 3-way bf16 split of E (6 products); for k > 4 the
+                              exp tile times X also on BF16 MFMA from 3-way splits (6 products, ~2^-24) */
 #define LAKER_PREC_FP64 2  /* Gram, exp and accumulation in FP64 (refinement residual) */
-#define LAKER_PREC_BF16 3  /* BF16 Gram and BF16 G.X products (probe matvec of the CCCP build) */
+#define LAKER_PREC_BF16 3  /* split Gram; for k > 4 one BF16 product of the exp tile and X (~2^-8) */
+#define LAKER_PREC_HILO 4  /* split Gram; for k > 4 hi/lo BF16 products (3 products, ~2^-16):
+                              the probe matvec of the mixed CCCP build */
 
 /* ---- clamp policy for exp(min(s*g, 80|700)) (laker/kernels.py:80-113,181-185) */
 #define LAKER_CLAMP_AUTO 0   /* clamp unless max_i ||e_i||^2 * scale < 80 (fp32) / 700 (fp64) */
@@ -131,7 +134,7 @@ int laker_cross_matvec_backward(laker_operator_t op, const float* Eq, int64_t m,
  * Replaces CCCPPreconditioner.build (laker/preconditioner.py:144-284).
  * probes: n x nr row-major in the operator dtype (device), or NULL to draw
  * N(0,1) probes on device from `seed`.  probe_precision selects the matvec
- * used for the probe responses (LAKER_PREC_FP32 or LAKER_PREC_BF16 ...).
+ * used for the probe responses (LAKER_PREC_FP32, _MIXED, _HILO or _BF16).
  * The QR is a shifted Cholesky-QR3 in FP64; the N_r x N_r CCCP loop, its
  * Cholesky factorisations and the final inverse square root run in FP64 on
  * device.  *iterations receives the CCCP iteration count. */

@@ -13,14 +13,19 @@ namespace laker {
 
 constexpr int GBM = 64, GBN = 64, GBK = 16;
 
-// One 64x64 tile of op(A) op(B) over the K range [k0, k1), accumulated in double.
+// One 64x64 tile of op(A) op(B) over the K range [k0, k1) on the FP64 MFMA
+// (v_mfma_f64_16x16x4f64).  Operands are converted to double while staged in
+// LDS (k-major); wave w owns the 32x32 quadrant (w / 2, w % 2) as 2x2 MFMA
+// 16x16 blocks.  Lane l supplies A[m = l % 16][k = l / 16], B[k = l / 16][n = l % 16]
+// and receives D[(l / 16) + 4 r][l % 16].
 template <class TA, class TB, bool TRA, bool TRB>
 __device__ inline void gemm_tile(const TA* __restrict__ A, int64_t lda, const TB* __restrict__ B,
                                  int64_t ldb, int64_t M, int64_t N, int64_t k0, int64_t k1,
-                                 int64_t m0, int64_t n0, double acc[4][4],
+                                 int64_t m0, int64_t n0, doublex4 acc[2][2],
                                  double (*As)[GBM + 1], double (*Bs)[GBN + 1]) {
   const int tid = threadIdx.x;
-  const int tm = tid / 16, tn = tid % 16;
+  const int lane = tid & 63, w = tid >> 6, wm = w >> 1, wn = w & 1;
+  const int lm = lane & 15, lk = lane >> 4;
   for (int64_t kb = k0; kb < k1; kb += GBK) {
     __syncthreads();
     for (int e = tid; e < GBK * GBM; e += 256) {
@@ -41,16 +46,16 @@ __device__ inline void gemm_tile(const TA* __restrict__ A, int64_t lda, const TB
     }
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < GBK; ++kk) {
-      double a[4], b[4];
+    for (int s4 = 0; s4 < GBK / 4; ++s4) {
+      double a[2], b[2];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = As[kk][tm + 16 * i];
+      for (int i = 0; i < 2; ++i) a[i] = As[4 * s4 + lk][32 * wm + 16 * i + lm];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tn + 16 * j];
+      for (int j = 0; j < 2; ++j) b[j] = Bs[4 * s4 + lk][32 * wn + 16 * j + lm];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
+        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
   }
 }
@@ -67,24 +72,31 @@ __global__ __launch_bounds__(256) void gemm_kernel(int64_t M, int64_t N, int64_t
   const int64_t m0 = (int64_t)blockIdx.y * GBM, n0 = (int64_t)blockIdx.x * GBN;
   const int64_t k0 = (int64_t)blockIdx.z * kchunk;
   const int64_t k1 = (k0 + kchunk < K) ? k0 + kchunk : K;
-  double acc[4][4] = {};
+  doublex4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = doublex4{0.0, 0.0, 0.0, 0.0};
   gemm_tile<TA, TB, TRA, TRB>(A, lda, B, ldb, M, N, k0, k1, m0, n0, acc, As, Bs);
-  const int tm = threadIdx.x / 16, tn = threadIdx.x % 16;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, wm = w >> 1, wn = w & 1;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int64_t gm = m0 + tm + 16 * i, gn = n0 + tn + 16 * j;
-      if (gm < M && gn < N) {
-        if (part) {
-          part[((int64_t)blockIdx.z * M + gm) * N + gn] = acc[i][j];
-        } else {
-          double v = alpha * acc[i][j];
-          if (beta != 0.0) v += beta * (double)C[gm * ldc + gn];
-          C[gm * ldc + gn] = (TC)v;
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gm = m0 + 32 * wm + 16 * i + (lane >> 4) + 4 * r;
+        const int64_t gn = n0 + 32 * wn + 16 * j + (lane & 15);
+        if (gm < M && gn < N) {
+          if (part) {
+            part[((int64_t)blockIdx.z * M + gm) * N + gn] = acc[i][j][r];
+          } else {
+            double v = alpha * acc[i][j][r];
+            if (beta != 0.0) v += beta * (double)C[gm * ldc + gn];
+            C[gm * ldc + gn] = (TC)v;
+          }
         }
       }
-    }
 }
 
 template <class TC>

@@ -839,31 +839,40 @@ __global__ __launch_bounds__(256) void mv_multi_kernel(
 //   X[32 t + jj(8u + v, lane/32)][c_base + 32 q + lane%32], v = 0..7.
 // ---------------------------------------------------------------------------
 __global__ void pack_x_kernel(const float* __restrict__ X, int64_t ldx, int64_t n_cols, int kcols,
-                              int c_base, int64_t ntiles, bf16x8* __restrict__ xh, bf16x8* __restrict__ xl) {
+                              int c_base, int64_t ntiles, bf16x8* __restrict__ xh, bf16x8* __restrict__ xl,
+                              bf16x8* __restrict__ xm) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= ntiles * 256) return;
   const int lane = (int)(idx & 63), u = (int)((idx >> 6) & 1), q = (int)((idx >> 7) & 1);
   const int64_t t = idx >> 8;
   const int h = lane >> 5, li = lane & 31;
   const int c = c_base + 32 * q + li;
-  bf16x8 hi, lo;
+  // x = hi + lo (hi/lo mode) or x = hi + mid + lo3 (xm given: 3-way split)
+  bf16x8 hi, lo, mid;
 #pragma unroll
   for (int v = 0; v < 8; ++v) {
     const int64_t j = t * 32 + jj_of(8 * u + v, h);
     const float x = (j < n_cols && c < kcols) ? X[j * ldx + c] : 0.f;
     const uint16_t hb = f32_to_bf16_bits(x);
+    const float r1 = x - bf16_bits_to_f32(hb);
+    const uint16_t mb = f32_to_bf16_bits(r1);
     hi[v] = __builtin_bit_cast(__bf16, hb);
-    lo[v] = __builtin_bit_cast(__bf16, f32_to_bf16_bits(x - bf16_bits_to_f32(hb)));
+    lo[v] = __builtin_bit_cast(__bf16, xm ? f32_to_bf16_bits(r1 - bf16_bits_to_f32(mb)) : mb);
+    mid[v] = __builtin_bit_cast(__bf16, mb);
   }
   xh[idx] = hi;
   if (xl) xl[idx] = lo;
+  if (xm) xm[idx] = mid;
 }
 
-template <int NS, int GX, bool CLAMP>
+constexpr int GX_SPLIT3 = 5;  // exp tile and X both split 3-way, 6 BF16 products (~2^-24)
+
+template <int NS, int GX, int NCC, bool CLAMP>
 __global__ __launch_bounds__(256) void mv_multi2_kernel(
     const void* __restrict__ colpack, const void* __restrict__ rowpack, int ka, int64_t n_rows,
-    int64_t n_cols, const bf16x8* __restrict__ xh, const bf16x8* __restrict__ xl, int kcols,
-    int c_base, float* __restrict__ part, int64_t ldp, int jsplit, float clampv) {
+    int64_t n_cols, const bf16x8* __restrict__ xh, const bf16x8* __restrict__ xl,
+    const bf16x8* __restrict__ xm, int kcols, int c_base, float* __restrict__ part, int64_t ldp,
+    int jsplit, float clampv) {
   using G = GramOps<LAKER_PREC_MIXED>;
   constexpr int ITW = 2;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, h = lane >> 5, li = lane & 31;
@@ -877,21 +886,24 @@ __global__ __launch_bounds__(256) void mv_multi2_kernel(
   for (int it = 0; it < ITW; ++it) G::load(B[it], rowpack, I0 + it * 32 + li, NS * 16, h);
   const int64_t ntiles = (n_cols + 31) / 32;
   const int64_t t0 = ntiles * js / jsplit, t1 = ntiles * (js + 1) / jsplit;
-  floatx16 acc[ITW][2];
+  floatx16 acc[ITW][NCC];
 #pragma unroll
-  for (int it = 0; it < ITW; ++it) acc[it][0] = acc[it][1] = floatx16{0};
+  for (int it = 0; it < ITW; ++it)
+#pragma unroll
+    for (int q = 0; q < NCC; ++q) acc[it][q] = floatx16{0};
 
   for (int64_t t = t0; t < t1; ++t) {
     G::Operand<NS> A;
     G::load(A, colpack, t * 32 + li, NS * 16, h);
-    bf16x8 Xh[2][2], Xl[2][2];
+    bf16x8 Xh[NCC][2], Xl[NCC][2], Xm[NCC][2];
 #pragma unroll
-    for (int q = 0; q < 2; ++q)
+    for (int q = 0; q < NCC; ++q)
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         const int64_t o = ((t * 2 + q) * 2 + u) * 64 + lane;
         Xh[q][u] = xh[o];
-        if constexpr (GX == LAKER_PREC_MIXED) Xl[q][u] = xl[o];
+        if constexpr (GX == LAKER_PREC_MIXED || GX == GX_SPLIT3) Xl[q][u] = xl[o];
+        if constexpr (GX == GX_SPLIT3) Xm[q][u] = xm[o];
       }
     floatx16 D[ITW];
 #pragma unroll
@@ -899,7 +911,7 @@ __global__ __launch_bounds__(256) void mv_multi2_kernel(
 #pragma unroll
     for (int it = 0; it < ITW; ++it) {
       // K ordering of chunk u: k = 8h + v  <->  j = jj(8u + v, h)
-      bf16x8 Ph[2], Pl[2];
+      bf16x8 Ph[2], Pl[2], Pm[2];
 #pragma unroll
       for (int u = 0; u < 2; ++u)
 #pragma unroll
@@ -909,11 +921,25 @@ __global__ __launch_bounds__(256) void mv_multi2_kernel(
           Ph[u][v] = __builtin_bit_cast(__bf16, hb);
           if constexpr (GX == LAKER_PREC_MIXED)
             Pl[u][v] = __builtin_bit_cast(__bf16, f32_to_bf16_bits(pv - bf16_bits_to_f32(hb)));
+          if constexpr (GX == GX_SPLIT3) {
+            const float r1 = pv - bf16_bits_to_f32(hb);
+            const uint16_t mb = f32_to_bf16_bits(r1);
+            Pm[u][v] = __builtin_bit_cast(__bf16, mb);
+            Pl[u][v] = __builtin_bit_cast(__bf16, f32_to_bf16_bits(r1 - bf16_bits_to_f32(mb)));
+          }
         }
 #pragma unroll
-      for (int q = 0; q < 2; ++q)
+      for (int q = 0; q < NCC; ++q)
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
+          if constexpr (GX == GX_SPLIT3) {
+            // smallest terms first: lo.hi + hi.lo + mid.mid + mid.hi + hi.mid + hi.hi
+            acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Pl[u], Xh[q][u], acc[it][q], 0, 0, 0);
+            acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ph[u], Xl[q][u], acc[it][q], 0, 0, 0);
+            acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Pm[u], Xm[q][u], acc[it][q], 0, 0, 0);
+            acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Pm[u], Xh[q][u], acc[it][q], 0, 0, 0);
+            acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ph[u], Xm[q][u], acc[it][q], 0, 0, 0);
+          }
           acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ph[u], Xh[q][u], acc[it][q], 0, 0, 0);
           if constexpr (GX == LAKER_PREC_MIXED) {
             acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Pl[u], Xh[q][u], acc[it][q], 0, 0, 0);
@@ -926,7 +952,7 @@ __global__ __launch_bounds__(256) void mv_multi2_kernel(
 #pragma unroll
   for (int it = 0; it < ITW; ++it)
 #pragma unroll
-    for (int q = 0; q < 2; ++q)
+    for (int q = 0; q < NCC; ++q)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int64_t i = I0 + it * 32 + jj_of(r, h);
@@ -1409,7 +1435,7 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
                       hipStream_t st, const int* skip) {
   LAKER_CHECK_ARG(k >= 1, "k must be >= 1");
   const float clampv = (float)(op.clamp_limit * kLog2eD);
-  const bool mixed = precision == LAKER_PREC_MIXED || precision == LAKER_PREC_BF16;
+  const bool mixed = precision == LAKER_PREC_MIXED || precision == LAKER_PREC_BF16 || precision == LAKER_PREC_HILO;
   const void* colpack = mixed ? cols.pack16 : cols.pack32;
   const void* rowpack = mixed ? rows.pack16 : rows.pack32;
   const int ka = mixed ? op.ka16 : op.ka32;
@@ -1454,9 +1480,12 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
     return LAKER_OK;
   }
   // multi-RHS on MFMA.  GX precision: FP32 path -> FP32 MFMA; MIXED -> hi/lo split; BF16 -> bf16.
+  // the right-hand-side product: FP32 MFMA; BF16 (1 product); hi/lo (3 products);
+  // 3-way split (6 products, the solver's mixed operator)
   const int gx = precision == LAKER_PREC_FP32 ? LAKER_PREC_FP32
                  : precision == LAKER_PREC_BF16 ? LAKER_PREC_BF16
-                                                : LAKER_PREC_MIXED;
+                 : precision == LAKER_PREC_HILO ? LAKER_PREC_MIXED
+                                                : GX_SPLIT3;
   // columns per launch: up to 64 (NCC=2) per pass; passes recompute the Gram tile.
   const int64_t kp = round_up(k, 32);
   LAKER_TRY(op.ensure_part((size_t)jsplit * n_rows * kp * sizeof(float)));
@@ -1465,26 +1494,32 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
   const bool packed_x = mixed && !deriv && gx != LAKER_PREC_FP32 && (ns16 == 2 || ns16 == 4 || ns16 == 6);
   if (packed_x) {
     const int64_t ntiles = ceil_div(n_cols, 32);
-    const size_t half = (size_t)ntiles * 256 * sizeof(bf16x8);
-    LAKER_TRY(op.xpack.ensure(2 * half));
+    const size_t third = (size_t)ntiles * 256 * sizeof(bf16x8);
+    LAKER_TRY(op.xpack.ensure(3 * third));
     bf16x8* xh = op.xpack.as<bf16x8>();
-    bf16x8* xl = reinterpret_cast<bf16x8*>(op.xpack.as<char>() + half);
-    const bool split = gx == LAKER_PREC_MIXED;
+    bf16x8* xl = reinterpret_cast<bf16x8*>(op.xpack.as<char>() + third);
+    bf16x8* xm = reinterpret_cast<bf16x8*>(op.xpack.as<char>() + 2 * third);
     dim3 grid((unsigned)(ceil_div(n_rows, 256) * jsplit)), block(256);
     for (int c0 = 0; c0 < k; c0 += 64) {
       hipLaunchKernelGGL(pack_x_kernel, dim3((unsigned)ceil_div(ntiles * 256, 256)), dim3(256), 0, st, X,
-                         ldx, n_cols, k, c0, ntiles, xh, split ? xl : nullptr);
+                         ldx, n_cols, k, c0, ntiles, xh, gx == LAKER_PREC_BF16 ? nullptr : xl,
+                         gx == GX_SPLIT3 ? xm : nullptr);
       LAKER_KCHECK("pack_x_kernel");
-#define LAKER_M2(NS, GXM)                                                                             \
+#define LAKER_M2C(NS, GXM, NC)                                                                        \
   do {                                                                                                \
     if (clamp)                                                                                        \
-      hipLaunchKernelGGL((mv_multi2_kernel<NS, GXM, true>), grid, block, 0, st, colpack, rowpack, ka,   \
-                         n_rows, n_cols, (const bf16x8*)xh, (const bf16x8*)xl, k, c0, part, kp, jsplit, \
-                         clampv);                                                                     \
+      hipLaunchKernelGGL((mv_multi2_kernel<NS, GXM, NC, true>), grid, block, 0, st, colpack, rowpack,   \
+                         ka, n_rows, n_cols, (const bf16x8*)xh, (const bf16x8*)xl, (const bf16x8*)xm, k, \
+                         c0, part, kp, jsplit, clampv);                                               \
     else                                                                                              \
-      hipLaunchKernelGGL((mv_multi2_kernel<NS, GXM, false>), grid, block, 0, st, colpack, rowpack, ka,  \
-                         n_rows, n_cols, (const bf16x8*)xh, (const bf16x8*)xl, k, c0, part, kp, jsplit, \
-                         clampv);                                                                     \
+      hipLaunchKernelGGL((mv_multi2_kernel<NS, GXM, NC, false>), grid, block, 0, st, colpack, rowpack,  \
+                         ka, n_rows, n_cols, (const bf16x8*)xh, (const bf16x8*)xl, (const bf16x8*)xm, k, \
+                         c0, part, kp, jsplit, clampv);                                               \
+  } while (0)
+#define LAKER_M2(NS, GXM)                              \
+  do {                                                 \
+    if (k - c0 > 32) LAKER_M2C(NS, GXM, 2);            \
+    else LAKER_M2C(NS, GXM, 1);                        \
   } while (0)
 #define LAKER_M2_NS(GXM)                     \
   do {                                       \
@@ -1492,10 +1527,12 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
     else if (ns16 == 4) LAKER_M2(4, GXM);    \
     else LAKER_M2(6, GXM);                   \
   } while (0)
-      if (split) LAKER_M2_NS(LAKER_PREC_MIXED);
+      if (gx == GX_SPLIT3) LAKER_M2_NS(GX_SPLIT3);
+      else if (gx == LAKER_PREC_MIXED) LAKER_M2_NS(LAKER_PREC_MIXED);
       else LAKER_M2_NS(LAKER_PREC_BF16);
 #undef LAKER_M2_NS
 #undef LAKER_M2
+#undef LAKER_M2C
       LAKER_KCHECK("mv_multi2_kernel");
     }
   } else {
@@ -1739,7 +1776,7 @@ int laker_matvec(laker_operator_t h, const void* X, int32_t k, int64_t ldx, void
                                      op->lambda_reg, (const double*)X, ldx, clamp, nullptr, stream);
   }
   LAKER_CHECK_ARG(precision == LAKER_PREC_FP32 || precision == LAKER_PREC_MIXED ||
-                      precision == LAKER_PREC_BF16,
+                      precision == LAKER_PREC_BF16 || precision == LAKER_PREC_HILO,
                   "bad precision");
   return fused_product_f32(*op, precision, rows, cols, (const float*)X, k, ldx, (float*)Y, ldy,
                            op->lambda_reg, (const float*)X, ldx, clamp, false, 0, stream);

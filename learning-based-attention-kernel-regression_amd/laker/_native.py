@@ -29,6 +29,7 @@ PREC_FP32 = 0
 PREC_MIXED = 1
 PREC_FP64 = 2
 PREC_BF16 = 3
+PREC_HILO = 4
 
 CLAMP_AUTO = 0
 CLAMP_ALWAYS = 1

@@ -24,7 +24,8 @@ from .backend import compute_device
 
 logger = logging.getLogger(__name__)
 
-_PRECISIONS = {"fp32": N.PREC_FP32, "mixed": N.PREC_MIXED, "bf16": N.PREC_BF16, "fp64": N.PREC_FP64}
+_PRECISIONS = {"fp32": N.PREC_FP32, "mixed": N.PREC_MIXED, "bf16": N.PREC_BF16, "hilo": N.PREC_HILO,
+               "fp64": N.PREC_FP64}
 
 
 class KernelOperator(Protocol):

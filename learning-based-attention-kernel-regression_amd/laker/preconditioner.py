@@ -31,7 +31,9 @@ from .utils import adaptive_shrinkage_rho, default_num_probes, eigh_stable
 
 logger = logging.getLogger(__name__)
 
-_PROBE_PREC = {"fp32": N.PREC_FP32, "bf16": N.PREC_BF16, "mixed": N.PREC_MIXED}
+# probe products: "mixed" = hi/lo BF16 products of the exp tile (~2^-16, enough
+# for the preconditioner), "split3" = the solver's FP32-level BF16 splitting
+_PROBE_PREC = {"fp32": N.PREC_FP32, "bf16": N.PREC_BF16, "mixed": N.PREC_HILO, "split3": N.PREC_MIXED}
 
 
 class _PrecondHandle:

@@ -14,7 +14,7 @@ import laker_oracle as O
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"fp32": 2e-5, "mixed": 3e-4, "bf16": 1e-2}
+TOL = {"fp32": 2e-5, "mixed": 2e-5, "hilo": 2e-4, "bf16": 1e-2}
 
 
 def scale_rows(e, lam, x, clamp=None):
@@ -44,7 +44,7 @@ CASES = [(1, 1, 1), (5, 3, 1), (31, 10, 1), (64, 10, 2), (100, 6, 3), (257, 16, 
 
 
 @pytest.mark.parametrize("n,d,k", CASES)
-@pytest.mark.parametrize("precision", ["fp32", "mixed", "bf16"])
+@pytest.mark.parametrize("precision", ["fp32", "mixed", "hilo", "bf16"])
 def test_matvec_matches_fp64(cuda, n, d, k, precision):
     e, x = make(n, d, k, seed=n + d + k)
     print(f"matvec n={n} d={d} k={k} precision={precision}", flush=True)
@@ -53,7 +53,9 @@ def test_matvec_matches_fp64(cuda, n, d, k, precision):
     yref = O.matvec_fp64(e, 0.05, x)
     s = scale_rows(e, 0.05, x)
     assert y.shape == x.shape
-    assert norm_err(y, yref, s) < TOL[precision]
+    # d > 16 with k > 4 has no packed 3-way kernel instance: hi/lo products
+    tol = TOL["hilo"] if (precision == "mixed" and d > 16 and k > 4) else TOL[precision]
+    assert norm_err(y, yref, s) < tol
 
 
 def test_matvec_fp64_operator(cuda):
