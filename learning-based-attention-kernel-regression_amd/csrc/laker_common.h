@@ -54,12 +54,10 @@ int fail(int code, const std::string& msg);
 inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
-// Round-to-nearest-even float -> bf16 bit pattern (finite inputs).
-__device__ __host__ inline uint16_t f32_to_bf16_bits(float f) {
-  uint32_t u = __builtin_bit_cast(uint32_t, f);
-  uint32_t lsb = (u >> 16) & 1u;
-  u += 0x7fffu + lsb;
-  return (uint16_t)(u >> 16);
+// Round-to-nearest-even float -> bf16 bit pattern (finite inputs): the
+// hardware conversion (v_cvt_pk_bf16_f32) in device code.
+__device__ inline uint16_t f32_to_bf16_bits(float f) {
+  return __builtin_bit_cast(uint16_t, static_cast<__bf16>(f));
 }
 __device__ __host__ inline float bf16_bits_to_f32(uint16_t b) {
   return __builtin_bit_cast(float, (uint32_t)b << 16);
