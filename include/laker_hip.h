@@ -217,6 +217,14 @@ int laker_solve_mixed(laker_operator_t op, laker_precond_t pc, const double* B, 
                       int32_t inner_max_iter, int32_t inner_precision, hipStream_t stream,
                       laker_mixed_result* result);
 
+/* HIP-graph replay of the device PCG iterations (laker_pcg_solve,
+ * laker_solve_mixed): after a first eager batch, each batch of polled
+ * iterations is one captured graph launch (iteration state lives on the
+ * device, so results are identical with and without graphs).  enable = 1 (the
+ * default) or 0; process-wide.  No reference counterpart: a launch-overhead
+ * option of the serving path. */
+int laker_set_graphs(int32_t enable);
+
 /* ---------------------------------------------------------------------------
  * Column-wise vector kernels for the generic PCG path, where operator or
  * preconditioner are Python callables (laker/solvers.py:225-286, 334-385).

@@ -10,6 +10,7 @@
 // Scalars live on device in FP64; vectors stay resident; the host only polls a
 // status word every `check_every` iterations, and every kernel after the
 // stopping point is a no-op.
+#include <atomic>
 #include <cmath>
 #include <vector>
 
@@ -26,12 +27,18 @@ enum { PCG_RUNNING = 0, PCG_CONVERGED = 1, PCG_BREAKDOWN = 2, PCG_MAXITER = 3 };
 
 struct PcgState {
   int32_t done;
-  int32_t iterations;
+  int32_t iterations;  // final count once done
+  int32_t it;          // iterations completed so far (device-side, so batches replay unchanged)
+  int32_t max_iter;
   double res_norm;
   double rhs_norm;
   double tol;
   double eps;
+  double xscale;       // x += xscale * alpha p in the refinement steps
 };
+
+// HIP-graph replay of batches of PCG iterations (on by default).
+static std::atomic<int> g_use_graphs{1};
 
 constexpr int DOT_TC = 64;   // columns per thread group (k > 1)
 constexpr int DOT_NB = 512;  // max partial blocks
@@ -105,7 +112,7 @@ __device__ inline double wave_sum_part(const double* part, int q, int nblk, int 
 // alpha = rz / pAp with the breakdown test.  One block, one wave per column.
 __global__ void pcg_alpha_kernel(const double* __restrict__ part, int nblk, int k,
                                  const double* __restrict__ rz, double* __restrict__ alpha,
-                                 PcgState* __restrict__ st, int it) {
+                                 PcgState* __restrict__ st) {
   if (st->done) return;
   __shared__ int brk;
   if (threadIdx.x == 0) brk = 0;
@@ -123,7 +130,7 @@ __global__ void pcg_alpha_kernel(const double* __restrict__ part, int nblk, int 
   __syncthreads();
   if (threadIdx.x == 0 && brk) {
     st->done = PCG_BREAKDOWN;
-    st->iterations = it;  // iterations completed before the breakdown
+    st->iterations = st->it;  // iterations completed before the breakdown
   }
 }
 
@@ -142,9 +149,10 @@ __global__ void pcg_update_xr_kernel(TX* __restrict__ x, T* __restrict__ r, cons
 }
 
 // residual norm, convergence and max-iteration bookkeeping.  One block.
-__global__ void pcg_resid_kernel(const double* __restrict__ part, int nblk, int k, int it,
-                                 int max_iter, PcgState* __restrict__ st) {
+__global__ void pcg_resid_kernel(const double* __restrict__ part, int nblk, int k,
+                                 PcgState* __restrict__ st) {
   if (st->done) return;
+  const int it = st->it, max_iter = st->max_iter;
   __shared__ double red[16];
   const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
   double s = 0.0;
@@ -160,6 +168,7 @@ __global__ void pcg_resid_kernel(const double* __restrict__ part, int nblk, int 
       st->done = PCG_MAXITER;
       st->iterations = max_iter;
     }
+    st->it = it + 1;
   }
 }
 
@@ -191,7 +200,8 @@ __global__ void pcg_update_p_kernel(T* __restrict__ p, const T* __restrict__ z,
 
 // set-up scalars: rz = colsum(r z), rhs_norm = ||b||_F.  One block.
 __global__ void pcg_setup_kernel(const double* __restrict__ part, int nblk, int k, double* __restrict__ rz,
-                                 PcgState* __restrict__ st, double tol, double eps) {
+                                 PcgState* __restrict__ st, double tol, double eps, int it0, int max_iter,
+                                 double xscale) {
   __shared__ double red[16];
   const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
   double s = 0.0;
@@ -203,11 +213,14 @@ __global__ void pcg_setup_kernel(const double* __restrict__ part, int nblk, int 
   s = block_sum(lane == 0 ? s : 0.0, red);
   if (threadIdx.x == 0) {
     st->done = PCG_RUNNING;
-    st->iterations = 0;
+    st->iterations = it0;
+    st->it = it0;
+    st->max_iter = max_iter;
     st->rhs_norm = sqrt(s);
     st->res_norm = INFINITY;
     st->tol = tol;
     st->eps = eps;
+    st->xscale = xscale;
   }
 }
 
@@ -302,6 +315,53 @@ static int pcg_buffers(Operator& op, int k, PcgBuffers<T>& b) {
   return LAKER_OK;
 }
 
+// Runs `count` PCG iterations given by `body`: eagerly, or by replaying a HIP
+// graph captured from one batch of `batch` iterations.  All per-iteration
+// state lives on the device (PcgState), so a captured batch replays
+// unchanged; launches after the stopping point are no-ops either way, so both
+// modes give identical results.  Capture problems fall back to eager launches.
+struct IterGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool failed = false;
+  IterGraph() = default;
+  IterGraph(const IterGraph&) = delete;
+  IterGraph& operator=(const IterGraph&) = delete;
+  ~IterGraph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+  template <class F>
+  int run(F&& body, int count, int batch, bool allow, hipStream_t st) {
+    if (allow && !failed && count == batch) {
+      if (!exec && hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        int rc = LAKER_OK;
+        for (int q = 0; q < batch && rc == LAKER_OK; ++q) rc = body();
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &g);
+        if (rc == LAKER_OK && e == hipSuccess && g != nullptr &&
+            hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) == hipSuccess) {
+          graph = g;
+        } else {
+          if (g) (void)hipGraphDestroy(g);
+          exec = nullptr;
+          failed = true;
+          (void)hipGetLastError();
+        }
+      } else if (!exec) {
+        failed = true;
+        (void)hipGetLastError();
+      }
+      if (exec) {
+        LAKER_HIP_CHECK(hipGraphLaunch(exec, st));
+        return LAKER_OK;
+      }
+    }
+    for (int q = 0; q < count; ++q) LAKER_TRY(body());
+    return LAKER_OK;
+  }
+};
+
 template <class T>
 static int pcg_run(Operator& op, Precond* pc, const T* B, int k, T* X, const T* X0, double tol,
                    int max_iter, double eps, int restart_freq, int precision, int check_every,
@@ -333,7 +393,8 @@ static int pcg_run(Operator& op, Precond* pc, const T* B, int k, T* X, const T* 
   hipLaunchKernelGGL((copy_kernel<T>), eg, eb, 0, st, (const T*)b.z, b.p, nk);
   LAKER_KCHECK("copy_kernel");
   LAKER_TRY(coldots<T>(b, 2, b.r, b.z, B, B, nullptr, nullptr, n, k, false, st));
-  hipLaunchKernelGGL(pcg_setup_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.st, tol, eps);
+  hipLaunchKernelGGL(pcg_setup_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.st, tol, eps, 0,
+                     max_iter, 1.0);
   LAKER_KCHECK("pcg_setup_kernel");
   LAKER_HIP_CHECK(hipGetLastError());
   PcgState hs;
@@ -357,24 +418,20 @@ static int pcg_run(Operator& op, Precond* pc, const T* B, int k, T* X, const T* 
     return LAKER_OK;
   }
   if (check_every < 1) check_every = 1;
-  for (int it = 0; it < max_iter; ++it) {
+  // one iteration: Ap, p.Ap (+ breakdown test), x/r update, ||r||, z = P r, beta, p
+  auto body = [&]() -> int {
     LAKER_TRY(op_apply<T>(op, precision, b.p, k, b.ap, &b.st->done, st));
     LAKER_TRY(coldots<T>(b, 3, b.p, b.ap, b.p, b.p, b.ap, b.ap, n, k, true, st));
-    hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.alpha, b.st, it);
+    hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.alpha, b.st);
     LAKER_KCHECK("pcg_alpha_kernel");
     hipLaunchKernelGGL((pcg_update_xr_kernel<T, T>), eg, eb, 0, st, b.x, b.r, b.p, b.ap, b.alpha, n, k,
                        b.st);
     LAKER_KCHECK("pcg_update_xr_kernel");
-    LAKER_HIP_CHECK(hipGetLastError());
-    if (restart_freq > 0 && (it + 1) % restart_freq == 0) {
-      // residual replacement r = b - A x
-      LAKER_TRY(op_apply<T>(op, precision, b.x, k, b.tmp, &b.st->done, st));
-      hipLaunchKernelGGL((axpby_kernel<T>), eg, eb, 0, st, B, 1.0, (const T*)b.tmp, -1.0, b.r, nk,
-                         (const PcgState*)b.st);
-      LAKER_KCHECK("axpby_kernel");
-    }
+    return LAKER_OK;
+  };
+  auto tail = [&]() -> int {
     LAKER_TRY(coldots<T>(b, 1, b.r, b.r, nullptr, nullptr, nullptr, nullptr, n, k, true, st));
-    hipLaunchKernelGGL(pcg_resid_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, it, max_iter, b.st);
+    hipLaunchKernelGGL(pcg_resid_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.st);
     LAKER_KCHECK("pcg_resid_kernel");
     if (pc) LAKER_TRY(precond_apply(*pc, b.r, k, k, b.z, k, st));
     else hipLaunchKernelGGL((copy_kernel<T>), eg, eb, 0, st, (const T*)b.r, b.z, nk);
@@ -383,12 +440,38 @@ static int pcg_run(Operator& op, Precond* pc, const T* B, int k, T* X, const T* 
     LAKER_KCHECK("pcg_beta_kernel");
     hipLaunchKernelGGL((pcg_update_p_kernel<T>), eg, eb, 0, st, b.p, b.z, b.beta, n, k, b.st);
     LAKER_KCHECK("pcg_update_p_kernel");
-    LAKER_HIP_CHECK(hipGetLastError());
-    if ((it + 1) % check_every == 0 || it + 1 == max_iter) {
-      LAKER_HIP_CHECK(hipMemcpyAsync(&hs, b.st, sizeof(hs), hipMemcpyDeviceToHost, st));
-      LAKER_HIP_CHECK(hipStreamSynchronize(st));
-      if (hs.done) break;
+    return LAKER_OK;
+  };
+  auto full = [&]() -> int {
+    LAKER_TRY(body());
+    return tail();
+  };
+  IterGraph ig;
+  const bool graphs = g_use_graphs.load() != 0 && restart_freq <= 0;
+  int launched = 0;
+  while (launched < max_iter) {
+    const int nb = std::min(check_every, max_iter - launched);
+    if (restart_freq > 0) {
+      for (int q = 0; q < nb; ++q) {
+        const int it = launched + q;
+        LAKER_TRY(body());
+        if ((it + 1) % restart_freq == 0) {
+          // residual replacement r = b - A x
+          LAKER_TRY(op_apply<T>(op, precision, b.x, k, b.tmp, &b.st->done, st));
+          hipLaunchKernelGGL((axpby_kernel<T>), eg, eb, 0, st, B, 1.0, (const T*)b.tmp, -1.0, b.r, nk,
+                             (const PcgState*)b.st);
+          LAKER_KCHECK("axpby_kernel");
+        }
+        LAKER_TRY(tail());
+      }
+    } else {
+      // the first batch runs eagerly (it sizes every work buffer), later ones replay
+      LAKER_TRY(ig.run(full, nb, check_every, graphs && launched > 0, st));
     }
+    launched += nb;
+    LAKER_HIP_CHECK(hipMemcpyAsync(&hs, b.st, sizeof(hs), hipMemcpyDeviceToHost, st));
+    LAKER_HIP_CHECK(hipStreamSynchronize(st));
+    if (hs.done) break;
   }
   LAKER_HIP_CHECK(hipMemcpyAsync(&hs, b.st, sizeof(hs), hipMemcpyDeviceToHost, st));
   LAKER_HIP_CHECK(hipStreamSynchronize(st));
@@ -441,11 +524,12 @@ __global__ void scale_to_f32_kernel(const double* __restrict__ a, float* __restr
 // x (FP64) += f * alpha p ; r -= alpha Ap  (FP32 recursion on a scaled residual)
 __global__ void pcg_update_x64_kernel(double* __restrict__ x, float* __restrict__ r,
                                       const float* __restrict__ p, const float* __restrict__ ap,
-                                      const double* __restrict__ alpha, double f, int64_t n, int k,
+                                      const double* __restrict__ alpha, int64_t n, int k,
                                       const PcgState* __restrict__ st) {
   if (st->done) return;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n * k) return;
+  const double f = st->xscale;
   const double a = alpha[idx % k];
   x[idx] += f * a * (double)p[idx];
   r[idx] = (float)((double)r[idx] - a * (double)ap[idx]);
@@ -458,6 +542,11 @@ __global__ void pcg_update_x64_kernel(double* __restrict__ x, float* __restrict_
 using namespace laker;
 
 extern "C" {
+
+int laker_set_graphs(int32_t enable) {
+  g_use_graphs.store(enable ? 1 : 0);
+  return LAKER_OK;
+}
 
 int laker_pcg_solve(laker_operator_t oh, laker_precond_t ph, const void* B, int32_t k, void* X,
                     const void* X0, double tol, int32_t max_iter, double breakdown_eps,
@@ -533,6 +622,31 @@ int laker_solve_mixed(laker_operator_t oh, laker_precond_t ph, const double* B, 
   int it = 0;
   const int check_every = 8;
   PcgState hs;
+  // one inner iteration; x accumulates in FP64 with the step's scale (device state)
+  auto body = [&]() -> int {
+    LAKER_TRY(op_apply<float>(op, inner_precision, b.p, k, b.ap, &b.st->done, st));
+    LAKER_TRY(coldots<float>(b, 3, b.p, b.ap, b.p, b.p, b.ap, b.ap, n, k, true, st));
+    hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.alpha, b.st);
+    LAKER_KCHECK("pcg_alpha_kernel");
+    hipLaunchKernelGGL(pcg_update_x64_kernel, eg, eb, 0, st, X, b.r, (const float*)b.p, (const float*)b.ap,
+                       (const double*)b.alpha, n, k, (const PcgState*)b.st);
+    LAKER_KCHECK("pcg_update_x64_kernel");
+    LAKER_TRY(coldots<float>(b, 1, b.r, b.r, nullptr, nullptr, nullptr, nullptr, n, k, true, st));
+    hipLaunchKernelGGL(pcg_resid_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.st);
+    LAKER_KCHECK("pcg_resid_kernel");
+    if (pc) LAKER_TRY(precond_apply(*pc, b.r, k, k, b.z, k, st));
+    else hipLaunchKernelGGL((copy_kernel<float>), eg, eb, 0, st, (const float*)b.r, b.z, nk);
+    LAKER_TRY(coldots<float>(b, 1, b.r, b.z, nullptr, nullptr, nullptr, nullptr, n, k, true, st));
+    hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.beta, b.st);
+    LAKER_KCHECK("pcg_beta_kernel");
+    hipLaunchKernelGGL((pcg_update_p_kernel<float>), eg, eb, 0, st, b.p, (const float*)b.z,
+                       (const double*)b.beta, n, k, (const PcgState*)b.st);
+    LAKER_KCHECK("pcg_update_p_kernel");
+    return LAKER_OK;
+  };
+  IterGraph ig;
+  const bool graphs = g_use_graphs.load() != 0;
+  bool eager_done = false;
   while (true) {
     const double need = 0.3 * tol * bnorm / rnorm;
     const double seg_tol = std::min(0.5, final_step ? need : std::max(inner_tol, need));
@@ -544,33 +658,18 @@ int laker_solve_mixed(laker_operator_t oh, laker_precond_t ph, const double* B, 
     LAKER_KCHECK("copy_kernel");
     LAKER_TRY(coldots<float>(b, 2, b.r, b.z, b.r, b.r, nullptr, nullptr, n, k, false, st));
     hipLaunchKernelGGL(pcg_setup_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.st, seg_tol,
-                       eps);
+                       eps, it, inner_max_iter, rnorm);
     LAKER_KCHECK("pcg_setup_kernel");
     const int seg_start = it;
     while (it < inner_max_iter) {
-      LAKER_TRY(op_apply<float>(op, inner_precision, b.p, k, b.ap, &b.st->done, st));
-      LAKER_TRY(coldots<float>(b, 3, b.p, b.ap, b.p, b.p, b.ap, b.ap, n, k, true, st));
-      hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.alpha, b.st, it);
-      LAKER_KCHECK("pcg_alpha_kernel");
-      hipLaunchKernelGGL(pcg_update_x64_kernel, eg, eb, 0, st, X, b.r, (const float*)b.p,
-                         (const float*)b.ap, (const double*)b.alpha, rnorm, n, k, (const PcgState*)b.st);
-      LAKER_KCHECK("pcg_update_x64_kernel");
-      LAKER_TRY(coldots<float>(b, 1, b.r, b.r, nullptr, nullptr, nullptr, nullptr, n, k, true, st));
-      hipLaunchKernelGGL(pcg_resid_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, it, inner_max_iter,
-                         b.st);
-      LAKER_KCHECK("pcg_resid_kernel");
-      if (pc) LAKER_TRY(precond_apply(*pc, b.r, k, k, b.z, k, st));
-      else hipLaunchKernelGGL((copy_kernel<float>), eg, eb, 0, st, (const float*)b.r, b.z, nk);
-      LAKER_TRY(coldots<float>(b, 1, b.r, b.z, nullptr, nullptr, nullptr, nullptr, n, k, true, st));
-      hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.beta, b.st);
-      LAKER_KCHECK("pcg_beta_kernel");
-      hipLaunchKernelGGL((pcg_update_p_kernel<float>), eg, eb, 0, st, b.p, (const float*)b.z,
-                         (const double*)b.beta, n, k, (const PcgState*)b.st);
-      LAKER_KCHECK("pcg_update_p_kernel");
-      ++it;
-      if (it % check_every != 0 && it < inner_max_iter) continue;
+      const int nb = std::min(check_every, inner_max_iter - it);
+      // the solve's first batch runs eagerly (it sizes every work buffer);
+      // later batches replay one captured graph (all state is on the device)
+      LAKER_TRY(ig.run(body, nb, check_every, graphs && eager_done, st));
+      eager_done = true;
       LAKER_HIP_CHECK(hipMemcpyAsync(&hs, b.st, sizeof(hs), hipMemcpyDeviceToHost, st));
       LAKER_HIP_CHECK(hipStreamSynchronize(st));
+      it = hs.it;
       if (hs.done != PCG_RUNNING) break;
     }
     LAKER_HIP_CHECK(hipMemcpyAsync(&hs, b.st, sizeof(hs), hipMemcpyDeviceToHost, st));

@@ -119,6 +119,7 @@ def _load():
                                         ctypes.POINTER(MixedResult)]),
         "laker_coldot": (c_int32, [c_void_p, c_void_p, c_int64, c_int32, c_int64, c_int32, c_void_p,
                                    c_void_p]),
+        "laker_set_graphs": (c_int32, [c_int32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -134,8 +135,9 @@ EXPORTED = [
     "laker_operator_info", "laker_operator_set_lambda", "laker_matvec", "laker_residual_fp64",
     "laker_cross_matvec", "laker_cross_matvec_t", "laker_kernel_eval", "laker_diagonal",
     "laker_to_dense", "laker_matvec_backward", "laker_cross_matvec_backward", "laker_precond_build",
-    "laker_precond_build_from_responses", "laker_precond_copy_out", "laker_precond_destroy", "laker_precond_info", "laker_precond_apply", "laker_pcg_solve",
-    "laker_solve_mixed", "laker_coldot",
+    "laker_precond_build_from_responses", "laker_precond_copy_out", "laker_precond_destroy",
+    "laker_precond_info", "laker_precond_apply", "laker_pcg_solve", "laker_solve_mixed",
+    "laker_set_graphs", "laker_coldot",
 ]
 
 
