@@ -867,87 +867,160 @@ __global__ void pack_x_kernel(const float* __restrict__ X, int64_t ldx, int64_t 
 
 constexpr int GX_SPLIT3 = 5;  // exp tile and X both split 3-way, 6 BF16 products (~2^-24)
 
-template <int NS, int GX, int NCC, bool CLAMP>
+template <int GX>
+struct M2Parts {
+  static constexpr int value = GX == GX_SPLIT3 ? 3 : (GX == LAKER_PREC_MIXED ? 2 : 1);
+};
+// LDS bytes of one chunk of JT column tiles: operand rows (padded) + X operands
+__host__ __device__ constexpr int m2_chunk_bytes(int ns, int nparts, int ncc, int jt) {
+  return jt * 32 * (32 * ns + 16) + jt * nparts * ncc * 2 * 64 * 16;
+}
+
+// Column operand rows and packed X operands of JT column tiles are staged
+// global -> registers -> LDS one chunk ahead (double buffer, one barrier per
+// chunk) and shared by the block's 4 waves.
+template <int NS, int GX, int NCC, int JT, int ITW, bool CLAMP>
 __global__ __launch_bounds__(256) void mv_multi2_kernel(
     const void* __restrict__ colpack, const void* __restrict__ rowpack, int ka, int64_t n_rows,
     int64_t n_cols, const bf16x8* __restrict__ xh, const bf16x8* __restrict__ xl,
     const bf16x8* __restrict__ xm, int kcols, int c_base, float* __restrict__ part, int64_t ldp,
     int jsplit, float clampv) {
   using G = GramOps<LAKER_PREC_MIXED>;
-  constexpr int ITW = 2;
+  constexpr int NP = M2Parts<GX>::value;
+  constexpr int gb = 32 * NS, lb = gb + 16, vpr = gb / 16;
+  constexpr int VA = JT * 32 * vpr;           // operand-row vectors per chunk
+  constexpr int XT = NP * NCC * 2 * 64;       // X vectors per tile
+  constexpr int VX = JT * XT;
+  constexpr int VPT = (VA + VX + 255) / 256;  // vectors per thread per chunk
+  constexpr int CHUNK = m2_chunk_bytes(NS, NP, NCC, JT);
+  extern __shared__ __align__(16) char m2_smem[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, h = lane >> 5, li = lane & 31;
   const int js = blockIdx.x % jsplit;
   const int64_t ib = blockIdx.x / jsplit;
   const int64_t I0 = (ib * 4 + w) * (32 * ITW);
-  if (I0 >= n_rows) return;
+  const bool active = I0 < n_rows;
   (void)ka;
   G::Operand<NS> B[ITW];
+  if (active) {
 #pragma unroll
-  for (int it = 0; it < ITW; ++it) G::load(B[it], rowpack, I0 + it * 32 + li, NS * 16, h);
+    for (int it = 0; it < ITW; ++it) G::load(B[it], rowpack, I0 + it * 32 + li, NS * 16, h);
+  }
   const int64_t ntiles = (n_cols + 31) / 32;
   const int64_t t0 = ntiles * js / jsplit, t1 = ntiles * (js + 1) / jsplit;
+  const char* gsrc = reinterpret_cast<const char*>(colpack);
+  const uint4* xsrc[3] = {reinterpret_cast<const uint4*>(xh), reinterpret_cast<const uint4*>(xl),
+                          reinterpret_cast<const uint4*>(xm)};
+  uint4 pre[VPT];
+  auto fetch = [&](int64_t t) {
+    const int nt = (int)((t1 - t) < JT ? (t1 - t) : JT);
+#pragma unroll
+    for (int u2 = 0; u2 < VPT; ++u2) {
+      const int vi = u2 * 256 + threadIdx.x;
+      uint4 v = {0u, 0u, 0u, 0u};
+      if (vi < VA) {
+        if (vi < nt * 32 * vpr) v = reinterpret_cast<const uint4*>(gsrc + t * 32 * gb)[vi];
+      } else if (vi < VA + VX) {
+        const int xi = vi - VA;
+        const int tt = xi / XT, rem = xi % XT;
+        const int pp = rem / (NCC * 2 * 64), off = rem % (NCC * 2 * 64);  // off = (q*2+u)*64 + lane
+        if (tt < nt) v = xsrc[pp][(t + tt) * 256 + off];
+      }
+      pre[u2] = v;
+    }
+  };
+  auto stash = [&](char* dst) {
+#pragma unroll
+    for (int u2 = 0; u2 < VPT; ++u2) {
+      const int vi = u2 * 256 + threadIdx.x;
+      if (vi < VA) {
+        *reinterpret_cast<uint4*>(dst + (vi / vpr) * lb + (vi % vpr) * 16) = pre[u2];
+      } else if (vi < VA + VX) {
+        *reinterpret_cast<uint4*>(dst + JT * 32 * lb + (vi - VA) * 16) = pre[u2];
+      }
+    }
+  };
   floatx16 acc[ITW][NCC];
 #pragma unroll
   for (int it = 0; it < ITW; ++it)
 #pragma unroll
     for (int q = 0; q < NCC; ++q) acc[it][q] = floatx16{0};
 
-  for (int64_t t = t0; t < t1; ++t) {
-    G::Operand<NS> A;
-    G::load(A, colpack, t * 32 + li, NS * 16, h);
-    bf16x8 Xh[NCC][2], Xl[NCC][2], Xm[NCC][2];
+  fetch(t0);
+  stash(m2_smem);
+  __syncthreads();
+  bool odd = false;
+  for (int64_t tc = t0; tc < t1; tc += JT) {
+    const bool more = tc + JT < t1;
+    if (more) fetch(tc + JT);
+    const int nt = (int)((t1 - tc) < JT ? (t1 - tc) : JT);
+    if (active) {
+      const char* cur = m2_smem + (odd ? CHUNK : 0);
+      const uint4* xs = reinterpret_cast<const uint4*>(cur + JT * 32 * lb);
+      for (int tt = 0; tt < nt; ++tt) {
+        G::Operand<NS> A;
+        const char* rowp = cur + (tt * 32 + li) * lb;
 #pragma unroll
-    for (int q = 0; q < NCC; ++q)
+        for (int q = 0; q < NS; ++q)
+          A.v[q] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const shortx8*>(rowp + q * 32 + h * 16));
+        bf16x8 Xh[NCC][2], Xl[NCC][2], Xm[NCC][2];
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const int64_t o = ((t * 2 + q) * 2 + u) * 64 + lane;
-        Xh[q][u] = xh[o];
-        if constexpr (GX == LAKER_PREC_MIXED || GX == GX_SPLIT3) Xl[q][u] = xl[o];
-        if constexpr (GX == GX_SPLIT3) Xm[q][u] = xm[o];
+        for (int q = 0; q < NCC; ++q)
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const int o = tt * XT + (q * 2 + u) * 64 + lane;
+            Xh[q][u] = __builtin_bit_cast(bf16x8, xs[o]);
+            if constexpr (NP >= 2) Xl[q][u] = __builtin_bit_cast(bf16x8, xs[o + NCC * 2 * 64]);
+            if constexpr (NP >= 3) Xm[q][u] = __builtin_bit_cast(bf16x8, xs[o + 2 * NCC * 2 * 64]);
+          }
+        floatx16 D[ITW];
+#pragma unroll
+        for (int it = 0; it < ITW; ++it) D[it] = G::gram_exact<NS>(A, B[it]);
+#pragma unroll
+        for (int it = 0; it < ITW; ++it) {
+          // K ordering of chunk u: k = 8h + v  <->  j = jj(8u + v, h)
+          bf16x8 Ph[2], Pl[2], Pm[2];
+#pragma unroll
+          for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+              const float pv = kexp<CLAMP, false>(D[it][8 * u + v], clampv);
+              const uint16_t hb = f32_to_bf16_bits(pv);
+              Ph[u][v] = __builtin_bit_cast(__bf16, hb);
+              if constexpr (GX == LAKER_PREC_MIXED)
+                Pl[u][v] = __builtin_bit_cast(__bf16, f32_to_bf16_bits(pv - bf16_bits_to_f32(hb)));
+              if constexpr (GX == GX_SPLIT3) {
+                const float r1 = pv - bf16_bits_to_f32(hb);
+                const uint16_t mb = f32_to_bf16_bits(r1);
+                Pm[u][v] = __builtin_bit_cast(__bf16, mb);
+                Pl[u][v] = __builtin_bit_cast(__bf16, f32_to_bf16_bits(r1 - bf16_bits_to_f32(mb)));
+              }
+            }
+#pragma unroll
+          for (int q = 0; q < NCC; ++q)
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+              if constexpr (GX == GX_SPLIT3) {
+                // smallest terms first: lo.hi + hi.lo + mid.mid + mid.hi + hi.mid + hi.hi
+                acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Pl[u], Xh[q][u], acc[it][q], 0, 0, 0);
+                acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ph[u], Xl[q][u], acc[it][q], 0, 0, 0);
+                acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Pm[u], Xm[q][u], acc[it][q], 0, 0, 0);
+                acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Pm[u], Xh[q][u], acc[it][q], 0, 0, 0);
+                acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ph[u], Xm[q][u], acc[it][q], 0, 0, 0);
+              }
+              acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ph[u], Xh[q][u], acc[it][q], 0, 0, 0);
+              if constexpr (GX == LAKER_PREC_MIXED) {
+                acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Pl[u], Xh[q][u], acc[it][q], 0, 0, 0);
+                acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ph[u], Xl[q][u], acc[it][q], 0, 0, 0);
+              }
+            }
+        }
       }
-    floatx16 D[ITW];
-#pragma unroll
-    for (int it = 0; it < ITW; ++it) D[it] = G::gram_exact<NS>(A, B[it]);
-#pragma unroll
-    for (int it = 0; it < ITW; ++it) {
-      // K ordering of chunk u: k = 8h + v  <->  j = jj(8u + v, h)
-      bf16x8 Ph[2], Pl[2], Pm[2];
-#pragma unroll
-      for (int u = 0; u < 2; ++u)
-#pragma unroll
-        for (int v = 0; v < 8; ++v) {
-          const float pv = kexp<CLAMP, false>(D[it][8 * u + v], clampv);
-          const uint16_t hb = f32_to_bf16_bits(pv);
-          Ph[u][v] = __builtin_bit_cast(__bf16, hb);
-          if constexpr (GX == LAKER_PREC_MIXED)
-            Pl[u][v] = __builtin_bit_cast(__bf16, f32_to_bf16_bits(pv - bf16_bits_to_f32(hb)));
-          if constexpr (GX == GX_SPLIT3) {
-            const float r1 = pv - bf16_bits_to_f32(hb);
-            const uint16_t mb = f32_to_bf16_bits(r1);
-            Pm[u][v] = __builtin_bit_cast(__bf16, mb);
-            Pl[u][v] = __builtin_bit_cast(__bf16, f32_to_bf16_bits(r1 - bf16_bits_to_f32(mb)));
-          }
-        }
-#pragma unroll
-      for (int q = 0; q < NCC; ++q)
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          if constexpr (GX == GX_SPLIT3) {
-            // smallest terms first: lo.hi + hi.lo + mid.mid + mid.hi + hi.mid + hi.hi
-            acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Pl[u], Xh[q][u], acc[it][q], 0, 0, 0);
-            acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ph[u], Xl[q][u], acc[it][q], 0, 0, 0);
-            acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Pm[u], Xm[q][u], acc[it][q], 0, 0, 0);
-            acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Pm[u], Xh[q][u], acc[it][q], 0, 0, 0);
-            acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ph[u], Xm[q][u], acc[it][q], 0, 0, 0);
-          }
-          acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ph[u], Xh[q][u], acc[it][q], 0, 0, 0);
-          if constexpr (GX == LAKER_PREC_MIXED) {
-            acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Pl[u], Xh[q][u], acc[it][q], 0, 0, 0);
-            acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ph[u], Xl[q][u], acc[it][q], 0, 0, 0);
-          }
-        }
     }
+    if (more) stash(m2_smem + (odd ? 0 : CHUNK));
+    __syncthreads();
+    odd = !odd;
   }
+  if (!active) return;
   // acc[it][q][r]: row i = I0 + it*32 + jj(r, h) ; column c = c_base + q*32 + li
 #pragma unroll
   for (int it = 0; it < ITW; ++it)
@@ -1499,22 +1572,32 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
     bf16x8* xh = op.xpack.as<bf16x8>();
     bf16x8* xl = reinterpret_cast<bf16x8*>(op.xpack.as<char>() + third);
     bf16x8* xm = reinterpret_cast<bf16x8*>(op.xpack.as<char>() + 2 * third);
-    dim3 grid((unsigned)(ceil_div(n_rows, 256) * jsplit)), block(256);
+    const int jsplit2 = jsplit;
+    const dim3 block(256);
     for (int c0 = 0; c0 < k; c0 += 64) {
       hipLaunchKernelGGL(pack_x_kernel, dim3((unsigned)ceil_div(ntiles * 256, 256)), dim3(256), 0, st, X,
                          ldx, n_cols, k, c0, ntiles, xh, gx == LAKER_PREC_BF16 ? nullptr : xl,
                          gx == GX_SPLIT3 ? xm : nullptr);
       LAKER_KCHECK("pack_x_kernel");
+#define LAKER_M2J(NS, GXM, NC, JTV)                                                                   \
+  do {                                                                                                \
+    constexpr size_t lds2 = 2 * (size_t)m2_chunk_bytes(NS, M2Parts<GXM>::value, NC, JTV);            \
+    /* one row tile per wave for the register-heavy 3-way split: higher occupancy */                 \
+    constexpr int ITWV = (GXM == GX_SPLIT3) ? 1 : 2;                                                  \
+    dim3 grid2((unsigned)(ceil_div(n_rows, 128 * ITWV) * jsplit2));                                   \
+    if (clamp)                                                                                        \
+      hipLaunchKernelGGL((mv_multi2_kernel<NS, GXM, NC, JTV, ITWV, true>), grid2, block, lds2, st,     \
+                         colpack, rowpack, ka, n_rows, n_cols, (const bf16x8*)xh, (const bf16x8*)xl,  \
+                         (const bf16x8*)xm, k, c0, part, kp, jsplit2, clampv);                        \
+    else                                                                                              \
+      hipLaunchKernelGGL((mv_multi2_kernel<NS, GXM, NC, JTV, ITWV, false>), grid2, block, lds2, st,    \
+                         colpack, rowpack, ka, n_rows, n_cols, (const bf16x8*)xh, (const bf16x8*)xl,  \
+                         (const bf16x8*)xm, k, c0, part, kp, jsplit2, clampv);                        \
+  } while (0)
 #define LAKER_M2C(NS, GXM, NC)                                                                        \
   do {                                                                                                \
-    if (clamp)                                                                                        \
-      hipLaunchKernelGGL((mv_multi2_kernel<NS, GXM, NC, true>), grid, block, 0, st, colpack, rowpack,   \
-                         ka, n_rows, n_cols, (const bf16x8*)xh, (const bf16x8*)xl, (const bf16x8*)xm, k, \
-                         c0, part, kp, jsplit, clampv);                                               \
-    else                                                                                              \
-      hipLaunchKernelGGL((mv_multi2_kernel<NS, GXM, NC, false>), grid, block, 0, st, colpack, rowpack,  \
-                         ka, n_rows, n_cols, (const bf16x8*)xh, (const bf16x8*)xl, (const bf16x8*)xm, k, \
-                         c0, part, kp, jsplit, clampv);                                               \
+    if (2 * m2_chunk_bytes(NS, M2Parts<GXM>::value, NC, 2) <= 65536) LAKER_M2J(NS, GXM, NC, 2);        \
+    else LAKER_M2J(NS, GXM, NC, 1);                                                                   \
   } while (0)
 #define LAKER_M2(NS, GXM)                              \
   do {                                                 \
@@ -1533,6 +1616,7 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
 #undef LAKER_M2_NS
 #undef LAKER_M2
 #undef LAKER_M2C
+#undef LAKER_M2J
       LAKER_KCHECK("mv_multi2_kernel");
     }
   } else {
