@@ -389,6 +389,131 @@ __global__ __launch_bounds__(256) void qw4_kernel(const float* __restrict__ Q, c
   }
 }
 
+// Q^T X for an FP32 Q and 2 <= k <= 32 columns:
+// part[s][c][j] = sum over the rows of split s of Q[i][c] X[i][j] (FP64),
+// c in the block's 32 columns.  v_mfma_f32_32x32x2f32 with lane (li, h)
+// supplying Q[i + h][c0 + li] (A[m = li][kk = h]) and X[i + h][li]
+// (B[kk = h][n = li]); FP32 accumulation over at most 64 row pairs, then FP64.
+// The four waves take interleaved row pairs and are combined in LDS in a fixed
+// order.
+__global__ __launch_bounds__(256) void qtx_mfma_kernel(const float* __restrict__ Q, int64_t n, int nr,
+                                                       const float* __restrict__ X, int64_t ldx, int k,
+                                                       int64_t rows_per_split, double* __restrict__ part) {
+  __shared__ double red[4][32][33];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 31, h = lane >> 5;
+  const int c0 = blockIdx.x * 32;
+  const int64_t s = blockIdx.y;
+  const int64_t r0 = s * rows_per_split;
+  const int64_t r1 = (r0 + rows_per_split < n) ? r0 + rows_per_split : n;
+  floatx16 acc = {0};
+  double acc64[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc64[r] = 0.0;
+  // eight row pairs per step: all loads first, then the MFMAs
+  constexpr int U = 8;
+  int cnt = 0;
+  for (int64_t i = r0 + 2 * w; i < r1; i += 8 * U) {
+    float a[U], b[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t row = i + 8 * u + h;
+      a[u] = (row < r1 && c0 + li < nr) ? Q[row * nr + c0 + li] : 0.f;
+      b[u] = (row < r1 && li < k) ? X[row * ldx + li] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], b[u], acc, 0, 0, 0);
+    cnt += U;
+    if (cnt >= 64) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc64[r] += (double)acc[r];
+      acc = floatx16{0};
+      cnt = 0;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    acc64[r] += (double)acc[r];
+    // D[m][n] at lane (li, h), register r: m = 8 (r/4) + 4 h + r%4 (Q column), n = li (X column)
+    red[w][8 * (r >> 2) + 4 * h + (r & 3)][li] = acc64[r];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+    const int m = e >> 5, j = e & 31;
+    if (c0 + m < nr && j < k)
+      part[(s * nr + c0 + m) * k + j] = (red[0][m][j] + red[1][m][j]) + (red[2][m][j] + red[3][m][j]);
+  }
+}
+
+// Z[i][j0 + j] = f X[i][j0 + j] + sum_c Q[i][c] W2[c][j0 + j] for j < 16 NB, FP32 Q, on
+// v_mfma_f32_16x16x4f32.  The block stages W2's columns j0 .. j0 + 16 NB - 1 in LDS
+// as FP32, transposed (row j, stride P floats, zero past nr), and each wave owns
+// QW_TILES tiles of 16 rows.  Lane (r = l % 16, g = l / 16) loads the quad
+// Q[i][c0 + 4g .. c0 + 4g + 3] of row i = i0 + r with one 16-byte load; MFMA step t
+// (t < 4) contracts over the columns c0 + 4g + t of the four lane groups:
+// A[m = r][kk = g] = Q[i0 + r][c0 + 4g + t] and B[kk = g][n = r] = W2[c0 + 4g + t][j0 + r],
+// the latter from one 16-byte LDS load per chunk of 16 columns.
+// D[m][n] sits at lane (n = l % 16, g = l / 16), register v: m = 4g + v.
+constexpr int QW_TILES = 2;
+constexpr int QW_BLOCK_ROWS = 4 * 16 * QW_TILES;
+static inline int qw_stride(int nr) {
+  const int nr16 = (nr + 15) & ~15;
+  return nr16 + (((4 - nr16) % 32) + 32) % 32;  // P = 4 (mod 32): conflict-free 16-byte reads
+}
+template <int NB>
+__global__ __launch_bounds__(256) void qw_mfma_kernel(const float* __restrict__ Q, const double* __restrict__ W2,
+                                                      int64_t n, int nr, int P, int k, int j0,
+                                                      const float* __restrict__ X, int64_t ldx,
+                                                      float* __restrict__ Z, int64_t ldz, double f) {
+  extern __shared__ float w2t[];  // 16 NB rows x P
+  constexpr int JW = 16 * NB;
+  for (int e = threadIdx.x; e < JW * P; e += blockDim.x) {
+    const int c = e / JW, jj = e % JW;
+    w2t[jj * P + c] = (c < nr && j0 + jj < k) ? (float)W2[(int64_t)c * k + j0 + jj] : 0.f;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+  const int64_t i0 = (int64_t)blockIdx.x * QW_BLOCK_ROWS + (threadIdx.x >> 6) * (16 * QW_TILES);
+  if (i0 >= n) return;
+  const float* qrow[QW_TILES];
+#pragma unroll
+  for (int tt = 0; tt < QW_TILES; ++tt) {
+    const int64_t i = i0 + 16 * tt + r;
+    qrow[tt] = Q + (i < n ? i : n - 1) * nr;
+  }
+  floatx4 acc[QW_TILES][NB];
+#pragma unroll
+  for (int tt = 0; tt < QW_TILES; ++tt)
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[tt][nb] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+  for (int c0 = 0; c0 < nr; c0 += 16) {
+    const int c = c0 + 4 * g;
+    floatx4 q[QW_TILES], w[NB];
+#pragma unroll
+    for (int tt = 0; tt < QW_TILES; ++tt)
+      q[tt] = (c < nr) ? *reinterpret_cast<const floatx4*>(qrow[tt] + c) : floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) w[nb] = *reinterpret_cast<const floatx4*>(w2t + (16 * nb + r) * P + c);
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int tt = 0; tt < QW_TILES; ++tt)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+          acc[tt][nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(q[tt][t], w[nb][t], acc[tt][nb], 0, 0, 0);
+  }
+#pragma unroll
+  for (int tt = 0; tt < QW_TILES; ++tt)
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t i = i0 + 16 * tt + 4 * g + v;
+        const int j = j0 + 16 * nb + r;
+        if (i < n && j < k) Z[i * ldz + j] = (float)(f * (double)X[i * ldx + j] + (double)acc[tt][nb][v]);
+      }
+}
+
 // out[c] = sum_b part[b][c]; block (64 columns x 4 row groups), fixed order per group
 __global__ __launch_bounds__(256) void reduce_parts_cols_kernel(const double* __restrict__ part, int nb, int nr,
                                                                 double* __restrict__ out) {
@@ -514,6 +639,46 @@ static int apply_t(Precond& pc, const T* X, int k, int64_t ldx, T* Z, int64_t ld
     LAKER_KCHECK("qw_vec_kernel");
     LAKER_HIP_CHECK(hipGetLastError());
     return LAKER_OK;
+  }
+  if constexpr (sizeof(T) == 4) {
+    if (k >= 2 && k <= 32 && nr % 4 == 0 && (size_t)16 * qw_stride(nr) * sizeof(float) <= 64 * 1024) {
+      // W1 = Q^T X on MFMA with split partials, two-level fixed-order reduction
+      const int cblk = (int)ceil_div(nr, 32);
+      const int64_t nsplit = std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), ceil_div(1024, cblk)));
+      const int64_t rps = ceil_div(n, nsplit);
+      const int64_t ns_eff = ceil_div(n, rps);
+      constexpr int RG = 8;
+      const int64_t width = (int64_t)nr * k;
+      LAKER_TRY(pc.part.ensure(((size_t)ns_eff + RG) * width * 8));
+      double* part = pc.part.as<double>();
+      double* part2 = part + (size_t)ns_eff * width;
+      hipLaunchKernelGGL(qtx_mfma_kernel, dim3((unsigned)cblk, (unsigned)ns_eff), dim3(256), 0, st, Q, n, nr, X,
+                         ldx, k, rps, part);
+      LAKER_KCHECK("qtx_mfma_kernel");
+      const unsigned wb = (unsigned)ceil_div(width, 64);
+      hipLaunchKernelGGL(reduce_parts_groups_kernel, dim3(wb, RG), dim3(256), 0, st, (const double*)part,
+                         (int)ns_eff, (int)width, part2);
+      LAKER_KCHECK("reduce_parts_groups_kernel");
+      hipLaunchKernelGGL(reduce_parts_groups_kernel, dim3(wb, 1), dim3(256), 0, st, (const double*)part2, RG,
+                         (int)width, W1);
+      LAKER_KCHECK("reduce_parts_groups_kernel");
+      LAKER_TRY((gemm<double, double, double>(false, false, nr, k, nr, 1.0, pc.S.as<double>(), nr, W1, k,
+                                               0.0, W2, k, nullptr, st)));
+      const int P = qw_stride(nr);
+      const unsigned qb = (unsigned)ceil_div(n, QW_BLOCK_ROWS);
+      if (k > 16 && (size_t)32 * P * sizeof(float) <= 64 * 1024) {
+        hipLaunchKernelGGL((qw_mfma_kernel<2>), dim3(qb), dim3(256), (size_t)32 * P * sizeof(float), st, Q,
+                           (const double*)W2, n, nr, P, k, 0, X, ldx, Z, ldz, pc.inv_sqrt_a);
+        LAKER_KCHECK("qw_mfma_kernel");
+      } else {
+        for (int j0 = 0; j0 < k; j0 += 16) {
+          hipLaunchKernelGGL((qw_mfma_kernel<1>), dim3(qb), dim3(256), (size_t)16 * P * sizeof(float), st, Q,
+                             (const double*)W2, n, nr, P, k, j0, X, ldx, Z, ldz, pc.inv_sqrt_a);
+          LAKER_KCHECK("qw_mfma_kernel");
+        }
+      }
+      return LAKER_OK;
+    }
   }
   // general k: W1 = Q^T X ; W2 = S W1 ; Z = a^{-1/2} X + Q W2
   LAKER_TRY((gemm<T, T, double>(true, false, nr, k, n, 1.0, Q, nr, X, ldx, 0.0, W1, k, &pc.ws, st)));

@@ -183,3 +183,23 @@ def test_multi_rhs_frobenius_rule(cuda, golden):
     assert abs(pcg.iterations - O.pcg_solve(
         lambda v: O.attention_matvec(torch.from_numpy(g["embeddings"]), 1e-2, v),
         lambda v: pre.apply(v.to(cuda)).cpu(), b.cpu(), tol=1e-5, max_iter=1000).iterations) <= 5
+
+
+@pytest.mark.parametrize("n,k", [(3000, 2), (3000, 7), (3000, 17), (3000, 32), (3000, 40), (72000, 20)])
+def test_precond_apply_multi_rhs_matches_columns(cuda, n, k):
+    """P X for several right-hand sides (MFMA Q^T X and MFMA Q W2 for k <= 32,
+    the FP64 GEMM path above) equals P applied column by column.  n = 72000
+    gives N_r = 536, whose 32 staged W2 columns exceed 64 KB of LDS, so Q W2
+    runs as two 16-column passes."""
+    x, _ = laker.pathloss_problem(n, seed=5)
+    emb = laker.PositionEmbedding(2, 10, device=cuda)
+    with torch.no_grad():
+        e = emb(x.to(cuda)).contiguous()
+    op = laker.AttentionKernelOperator(e, lambda_reg=1e-2)
+    pre = laker.CCCPPreconditioner(gamma=0.1, verbose=False, probe_source="device").build(op.matvec, n, seed=1)
+    xs = torch.randn(n, k, generator=torch.Generator().manual_seed(k)).to(cuda)
+    z = pre.apply(xs)
+    cols = torch.stack([pre.apply(xs[:, j].contiguous()) for j in range(k)], dim=1)
+    err = float(((z - cols).abs().max() / cols.abs().max()))
+    print(f"apply n={n} k={k} N_r={pre.nr}: max rel diff vs per-column {err:.2e}", flush=True)
+    assert z.shape == xs.shape and err < 2e-6
