@@ -164,6 +164,40 @@ int laker_precond_info(laker_precond_t pc, double* isotropic_coef, int32_t* nr, 
  * (laker/preconditioner.py:52-68,286-352). */
 int laker_precond_apply(laker_precond_t pc, const void* X, int32_t k, int64_t ldx, void* Z,
                         int64_t ldz, hipStream_t stream);
+/* Jacobi preconditioner P = diag(lambda I + G)^{-1}: the handle stores
+ * inv = 1 / max(diag, eps(dtype)) computed in dtype (LAKER_F32/F64) from the
+ * device diagonal (n entries), and laker_precond_apply then returns
+ * Z[i][c] = inv[i] * X[i][c] for every column, so it also drives
+ * laker_pcg_solve and laker_solve_mixed.  copy_out(which = 4) copies inv out;
+ * laker_precond_info rejects it.  Replaces JacobiPreconditioner.__init__ /
+ * apply (laker/solvers.py:505-546). */
+int laker_precond_create_jacobi(const void* diagonal, int64_t n, int32_t dtype, hipStream_t stream,
+                                laker_precond_t* out);
+
+/* A CCCP handle from stored factors, without the probe matvec or the CCCP
+ * loop: Q (n x nr, dtype), R, C and S = M^{-1/2} - a^{-1/2} I (nr x nr FP64,
+ * as laker_precond_copy_out returns them), isotropic_coef a > 0 and rho; all
+ * device pointers, copied into the handle.  The reference's save format holds
+ * no preconditioner (laker/persistence.py:74-328); this backs the compatible
+ * extension that stores the factors, so a loaded model solves at once. */
+int laker_precond_from_factors(const void* Q, int64_t n, int32_t nr, int32_t dtype, double isotropic_coef,
+                               double rho, const double* R, const double* C, const double* S,
+                               hipStream_t stream, laker_precond_t* out);
+
+/* In-place orthonormal basis of the n x k block X (row-major, ld = k, device,
+ * LAKER_F32/F64, n >= k): columns scaled to unit norm, then Cholesky-QR2 in
+ * FP64 (shifted Cholesky-QR3 if the first Gram factorisation fails).  The
+ * result equals the Q of a Householder QR up to column signs.  LAKER_ENUMERIC
+ * for a numerically rank-deficient block.  Replaces the torch.linalg.qr of
+ * the "power_iter" probe strategy (laker/preconditioner.py:174-184). */
+int laker_orthonormalize(void* X, int64_t n, int32_t k, int32_t dtype, hipStream_t stream);
+
+/* count N(0,1) draws (LAKER_F32/F64) into device memory: the same stream
+ * laker_precond_build draws when it is given no probe matrix, so a caller
+ * can transform those probes first (the "power_iter" strategy) and pass them
+ * back.  Replaces the device torch.randn of laker/preconditioner.py:168-171
+ * for probe_source="device" (a different generator than torch's). */
+int laker_randn(void* out, int64_t count, int32_t dtype, uint64_t seed, hipStream_t stream);
 
 /* ---------------------------------------------------------------------------
  * Device-resident PCG.  Replaces PreconditionedConjugateGradient.solve /

@@ -20,14 +20,21 @@
 
 namespace laker {
 
+enum PrecondKind { PRECOND_CCCP = 0, PRECOND_JACOBI = 1 };
+
 struct Precond {
+  int kind = PRECOND_CCCP;
   int64_t n = 0;
   int nr = 0;
   int dtype = LAKER_F32;
   double a = 1.0, rho = 0.0, inv_sqrt_a = 1.0;
   DevBuf Q, R, C, S;
   DevBuf w1, w2, part, ws, scal;
+  DevBuf inv_diag;  // Jacobi: 1 / max(diag, eps(dtype)), n entries of dtype
 };
+
+int64_t precond_rows(const Precond& pc) { return pc.n; }
+int precond_dtype(const Precond& pc) { return pc.dtype; }
 
 // ---------------------------------------------------------------------------
 // small kernels
@@ -87,6 +94,26 @@ __global__ void convert_kernel(const TI* __restrict__ in, TO* __restrict__ out, 
 __global__ void add_diag_kernel(double* __restrict__ A, int64_t n, double v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) A[i * n + i] += v;
+}
+
+// Jacobi preconditioner (laker/solvers.py:505-546): inv = 1 / max(d, eps) in the
+// solver dtype (a NaN diagonal stays NaN, as torch's clamp leaves it), and
+// Z = inv * X row-wise for every column.
+template <class T>
+__global__ void jacobi_inv_kernel(const T* __restrict__ diag, T* __restrict__ inv, int64_t n, T eps) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T d = diag[i];
+  inv[i] = T(1) / (d < eps ? eps : d);
+}
+template <class T>
+__global__ void jacobi_apply_kernel(const T* __restrict__ inv, const T* __restrict__ X, int64_t ldx,
+                                    T* __restrict__ Z, int64_t ldz, int64_t n, int k) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * k) return;
+  const int64_t i = idx / k;
+  const int c = (int)(idx % k);
+  Z[i * ldz + c] = inv[i] * X[i * ldx + c];
 }
 
 // M = C + a I
@@ -693,6 +720,17 @@ static int apply_t(Precond& pc, const T* X, int k, int64_t ldx, T* Z, int64_t ld
 }
 
 int precond_apply(Precond& pc, const void* X, int k, int64_t ldx, void* Z, int64_t ldz, hipStream_t st) {
+  if (pc.kind == PRECOND_JACOBI) {
+    const dim3 g((unsigned)ceil_div(pc.n * k, 256)), b(256);
+    if (pc.dtype == LAKER_F32)
+      hipLaunchKernelGGL((jacobi_apply_kernel<float>), g, b, 0, st, pc.inv_diag.as<float>(), (const float*)X, ldx,
+                         (float*)Z, ldz, pc.n, k);
+    else
+      hipLaunchKernelGGL((jacobi_apply_kernel<double>), g, b, 0, st, pc.inv_diag.as<double>(), (const double*)X,
+                         ldx, (double*)Z, ldz, pc.n, k);
+    LAKER_KCHECK("jacobi_apply_kernel");
+    return LAKER_OK;
+  }
   if (pc.dtype == LAKER_F32) return apply_t<float>(pc, (const float*)X, k, ldx, (float*)Z, ldz, st);
   return apply_t<double>(pc, (const double*)X, k, ldx, (double*)Z, ldz, st);
 }
@@ -733,6 +771,49 @@ static int cholqr_pass(double* V, int64_t n, int nr, double shift, double* G, do
   return LAKER_OK;
 }
 
+// Orthonormal basis of the n x nr block src (dtype LAKER_F32/F64) in FP64:
+// Cholesky-QR2 (two passes), or a shifted Cholesky-QR3 when the first Gram
+// matrix is not numerically positive definite.  V receives the basis, Racc the
+// accumulated upper-triangular factor (src = V Racc).  LAKER_ENUMERIC if every
+// route fails (rank-deficient block).
+static int cholqr_block(const void* src, int dtype, int64_t n, int nr, double* V, double* Racc, DevBuf* ws,
+                        hipStream_t st) {
+  const int64_t nn = (int64_t)nr * nr;
+  DevBuf Vtmp, G, Linv, tmp, infob;
+  TriWork tri;
+  LAKER_TRY(Vtmp.ensure((size_t)n * nr * 8));
+  LAKER_TRY(G.ensure(nn * 8));
+  LAKER_TRY(Linv.ensure(nn * 8));
+  LAKER_TRY(tmp.ensure(nn * 8));
+  LAKER_TRY(infob.ensure(sizeof(int)));
+  auto load_v = [&]() -> int {
+    if (dtype == LAKER_F32)
+      hipLaunchKernelGGL((convert_kernel<float, double>), dim3((unsigned)ceil_div(n * nr, 256)), dim3(256), 0,
+                         st, (const float*)src, V, n * nr);
+    else
+      LAKER_HIP_CHECK(hipMemcpyAsync(V, src, (size_t)n * nr * 8, hipMemcpyDeviceToDevice, st));
+    LAKER_HIP_CHECK(hipGetLastError());
+    return set_identity(Racc, nr, 1.0, st);
+  };
+  auto pass = [&](double shift) {
+    return cholqr_pass(V, n, nr, shift, G.as<double>(), Linv.as<double>(), Racc, tmp.as<double>(),
+                       Vtmp.as<double>(), infob.as<int>(), ws, tri, st);
+  };
+  LAKER_TRY(load_v());
+  int rc = pass(0.0);
+  if (rc == LAKER_ENUMERIC) {
+    // shifted Cholesky-QR: shift = 11 (n nr + nr (nr + 1)) u ||U||_F^2, ||U||_F^2 <= nr
+    // (columns of unit norm; callers normalise first)
+    LAKER_TRY(load_v());
+    const double u = 1.1102230246251565e-16;
+    const double shift = 11.0 * ((double)n * nr + (double)nr * (nr + 1)) * u * (double)nr;
+    rc = pass(shift);
+    if (rc == LAKER_OK) rc = pass(0.0);
+  }
+  if (rc == LAKER_OK) rc = pass(0.0);
+  return rc;
+}
+
 // Steps 2-5 of the build on probe responses U (n x nr, op dtype; consumed).
 static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, int nr, double gamma,
                         double epsilon, double base_rho, int max_iter, double tol, hipStream_t st,
@@ -760,45 +841,15 @@ static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, i
   LAKER_HIP_CHECK(hipGetLastError());
 
   // 3. Cholesky-QR2 in FP64 (shifted Cholesky-QR3 on failure)
-  DevBuf V, Vtmp, G, Linv, Racc, tmp, infob;
-  TriWork tri;
+  DevBuf V, Racc;
   LAKER_TRY(V.ensure((size_t)n * nr * 8));
-  LAKER_TRY(Vtmp.ensure((size_t)n * nr * 8));
-  LAKER_TRY(G.ensure(nn * 8));
-  LAKER_TRY(Linv.ensure(nn * 8));
   LAKER_TRY(Racc.ensure(nn * 8));
-  LAKER_TRY(tmp.ensure(nn * 8));
-  LAKER_TRY(infob.ensure(sizeof(int)));
-  auto load_v = [&]() -> int {
-    if (pc->dtype == LAKER_F32)
-      hipLaunchKernelGGL((convert_kernel<float, double>), dim3((unsigned)ceil_div(n * nr, 256)), dim3(256), 0,
-                         st, U.as<float>(), V.as<double>(), n * nr);
-    else
-      LAKER_HIP_CHECK(hipMemcpyAsync(V.ptr, U.ptr, (size_t)n * nr * 8, hipMemcpyDeviceToDevice, st));
-    LAKER_HIP_CHECK(hipGetLastError());
-    return set_identity(Racc.as<double>(), nr, 1.0, st);
-  };
-  LAKER_TRY(load_v());
-  int rc = cholqr_pass(V.as<double>(), n, nr, 0.0, G.as<double>(), Linv.as<double>(), Racc.as<double>(),
-                       tmp.as<double>(), Vtmp.as<double>(), infob.as<int>(), &pc->ws, tri, st);
-  if (rc == LAKER_ENUMERIC) {
-    // shifted Cholesky-QR: shift = 11 (n nr + nr (nr + 1)) u ||U||_F^2, ||U||_F^2 <= nr
-    LAKER_TRY(load_v());
-    const double u = 1.1102230246251565e-16;
-    const double shift = 11.0 * ((double)n * nr + (double)nr * (nr + 1)) * u * (double)nr;
-    rc = cholqr_pass(V.as<double>(), n, nr, shift, G.as<double>(), Linv.as<double>(), Racc.as<double>(),
-                     tmp.as<double>(), Vtmp.as<double>(), infob.as<int>(), &pc->ws, tri, st);
-    if (rc == LAKER_OK)
-      rc = cholqr_pass(V.as<double>(), n, nr, 0.0, G.as<double>(), Linv.as<double>(), Racc.as<double>(),
-                       tmp.as<double>(), Vtmp.as<double>(), infob.as<int>(), &pc->ws, tri, st);
+  {
+    const int rc = cholqr_block(U.ptr, pc->dtype, n, nr, V.as<double>(), Racc.as<double>(), &pc->ws, st);
+    if (rc == LAKER_ENUMERIC) return fail(LAKER_ENUMERIC, "CCCP build: Cholesky-QR of the probe responses failed");
+    LAKER_TRY(rc);
   }
-  if (rc == LAKER_OK)
-    rc = cholqr_pass(V.as<double>(), n, nr, 0.0, G.as<double>(), Linv.as<double>(), Racc.as<double>(),
-                     tmp.as<double>(), Vtmp.as<double>(), infob.as<int>(), &pc->ws, tri, st);
-  if (rc == LAKER_ENUMERIC) return fail(LAKER_ENUMERIC, "CCCP build: Cholesky-QR of the probe responses failed");
-  LAKER_TRY(rc);
   U.release();
-  Vtmp.release();
   LAKER_TRY(pc->Q.ensure((size_t)n * nr * esz));
   if (pc->dtype == LAKER_F32)
     hipLaunchKernelGGL((convert_kernel<double, float>), dim3((unsigned)ceil_div(n * nr, 256)), dim3(256), 0,
@@ -823,7 +874,10 @@ static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, i
   pc->rho = rho;
   const double s = 1.0 + gamma / (double)n;
   const double a_rho = (1.0 - rho) * gamma / s + rho;
-  DevBuf Cb, Cn, Mb, Zb, Rw, Hb, scb, partb;
+  DevBuf Cb, Cn, Mb, Zb, Rw, Hb, scb, partb, infob, Linv;
+  TriWork tri;
+  LAKER_TRY(infob.ensure(sizeof(int)));
+  LAKER_TRY(Linv.ensure(nn * 8));
   LAKER_TRY(Cb.ensure(nn * 8));
   LAKER_TRY(Cn.ensure(nn * 8));
   LAKER_TRY(Mb.ensure(nn * 8));
@@ -1019,9 +1073,111 @@ int laker_precond_build_from_responses(const void* U_in, int64_t n, int32_t nr, 
   return LAKER_OK;
 }
 
+int laker_precond_create_jacobi(const void* diagonal, int64_t n, int32_t dtype, hipStream_t st,
+                                laker_precond_t* out) {
+  LAKER_CHECK_ARG(diagonal != nullptr && out != nullptr && n >= 1, "bad Jacobi preconditioner arguments");
+  LAKER_CHECK_ARG(dtype == LAKER_F32 || dtype == LAKER_F64, "dtype must be LAKER_F32 or LAKER_F64");
+  std::unique_ptr<Precond> pc(new Precond());
+  pc->kind = PRECOND_JACOBI;
+  pc->n = n;
+  pc->nr = 0;
+  pc->dtype = dtype;
+  const dim3 g((unsigned)ceil_div(n, 256)), b(256);
+  if (dtype == LAKER_F32) {
+    LAKER_TRY(pc->inv_diag.ensure((size_t)n * 4));
+    hipLaunchKernelGGL((jacobi_inv_kernel<float>), g, b, 0, st, (const float*)diagonal, pc->inv_diag.as<float>(), n,
+                       1.1920928955078125e-07f);
+  } else {
+    LAKER_TRY(pc->inv_diag.ensure((size_t)n * 8));
+    hipLaunchKernelGGL((jacobi_inv_kernel<double>), g, b, 0, st, (const double*)diagonal,
+                       pc->inv_diag.as<double>(), n, 2.220446049250313e-16);
+  }
+  LAKER_KCHECK("jacobi_inv_kernel");
+  *out = reinterpret_cast<laker_precond_t>(pc.release());
+  return LAKER_OK;
+}
+
+int laker_precond_from_factors(const void* Q, int64_t n, int32_t nr, int32_t dtype, double isotropic_coef,
+                               double rho, const double* R, const double* C, const double* S, hipStream_t st,
+                               laker_precond_t* out) {
+  LAKER_CHECK_ARG(Q && R && C && S && out, "from_factors needs Q, R, C, S and an out handle");
+  LAKER_CHECK_ARG(nr >= 1 && nr <= n && nr <= 4096, "num_probes must be in [1, min(n, 4096)]");
+  LAKER_CHECK_ARG(dtype == LAKER_F32 || dtype == LAKER_F64, "dtype must be LAKER_F32 or LAKER_F64");
+  LAKER_CHECK_ARG(isotropic_coef > 0.0 && std::isfinite(isotropic_coef), "isotropic_coef must be positive");
+  std::unique_ptr<Precond> pc(new Precond());
+  pc->n = n;
+  pc->nr = nr;
+  pc->dtype = dtype;
+  pc->a = isotropic_coef;
+  pc->rho = rho;
+  pc->inv_sqrt_a = 1.0 / std::sqrt(isotropic_coef);
+  const size_t qb = (size_t)n * nr * (dtype == LAKER_F32 ? 4 : 8), nn8 = (size_t)nr * nr * 8;
+  LAKER_TRY(pc->Q.ensure(qb));
+  LAKER_TRY(pc->R.ensure(nn8));
+  LAKER_TRY(pc->C.ensure(nn8));
+  LAKER_TRY(pc->S.ensure(nn8));
+  LAKER_HIP_CHECK(hipMemcpyAsync(pc->Q.ptr, Q, qb, hipMemcpyDeviceToDevice, st));
+  LAKER_HIP_CHECK(hipMemcpyAsync(pc->R.ptr, R, nn8, hipMemcpyDeviceToDevice, st));
+  LAKER_HIP_CHECK(hipMemcpyAsync(pc->C.ptr, C, nn8, hipMemcpyDeviceToDevice, st));
+  LAKER_HIP_CHECK(hipMemcpyAsync(pc->S.ptr, S, nn8, hipMemcpyDeviceToDevice, st));
+  *out = reinterpret_cast<laker_precond_t>(pc.release());
+  return LAKER_OK;
+}
+
+int laker_orthonormalize(void* X, int64_t n, int32_t k, int32_t dtype, hipStream_t st) {
+  LAKER_CHECK_ARG(X != nullptr && k >= 1 && n >= k, "orthonormalize needs an n x k block with n >= k >= 1");
+  LAKER_CHECK_ARG(k <= 4096, "orthonormalize: k > 4096 not supported");
+  LAKER_CHECK_ARG(dtype == LAKER_F32 || dtype == LAKER_F64, "dtype must be LAKER_F32 or LAKER_F64");
+  const int64_t nk = n * k;
+  const dim3 eg((unsigned)ceil_div(nk, 256)), eb(256);
+  DevBuf W, ss, V, Racc, ws;
+  LAKER_TRY(W.ensure((size_t)nk * 8));
+  LAKER_TRY(ss.ensure((size_t)k * 8));
+  LAKER_TRY(V.ensure((size_t)nk * 8));
+  LAKER_TRY(Racc.ensure((size_t)k * k * 8));
+  // FP64 copy with unit columns (Q is unchanged by a positive column scaling)
+  if (dtype == LAKER_F32)
+    hipLaunchKernelGGL((convert_kernel<float, double>), eg, eb, 0, st, (const float*)X, W.as<double>(), nk);
+  else
+    LAKER_HIP_CHECK(hipMemcpyAsync(W.ptr, X, (size_t)nk * 8, hipMemcpyDeviceToDevice, st));
+  LAKER_KCHECK("convert_kernel");
+  hipLaunchKernelGGL((colsumsq_kernel<double>), dim3(k), dim3(256), 0, st, W.as<double>(), n, k, ss.as<double>());
+  LAKER_KCHECK("colsumsq_kernel");
+  hipLaunchKernelGGL((colscale_kernel<double>), eg, eb, 0, st, W.as<double>(), n, k, ss.as<double>(), 1e-300);
+  LAKER_KCHECK("colscale_kernel");
+  const int rc = cholqr_block(W.ptr, LAKER_F64, n, k, V.as<double>(), Racc.as<double>(), &ws, st);
+  if (rc == LAKER_ENUMERIC) return fail(LAKER_ENUMERIC, "orthonormalize: the block is numerically rank-deficient");
+  LAKER_TRY(rc);
+  if (dtype == LAKER_F32)
+    hipLaunchKernelGGL((convert_kernel<double, float>), eg, eb, 0, st, V.as<double>(), (float*)X, nk);
+  else
+    LAKER_HIP_CHECK(hipMemcpyAsync(X, V.ptr, (size_t)nk * 8, hipMemcpyDeviceToDevice, st));
+  LAKER_KCHECK("convert_kernel");
+  return LAKER_OK;
+}
+
+int laker_randn(void* out, int64_t count, int32_t dtype, uint64_t seed, hipStream_t st) {
+  LAKER_CHECK_ARG(out != nullptr && count >= 0, "bad randn arguments");
+  LAKER_CHECK_ARG(dtype == LAKER_F32 || dtype == LAKER_F64, "dtype must be LAKER_F32 or LAKER_F64");
+  if (count == 0) return LAKER_OK;
+  const dim3 g((unsigned)ceil_div(count, 256)), b(256);
+  if (dtype == LAKER_F32)
+    hipLaunchKernelGGL((randn_kernel<float>), g, b, 0, st, (float*)out, count, seed);
+  else
+    hipLaunchKernelGGL((randn_kernel<double>), g, b, 0, st, (double*)out, count, seed);
+  LAKER_KCHECK("randn_kernel");
+  return LAKER_OK;
+}
+
 int laker_precond_copy_out(laker_precond_t h, int32_t which, void* dst, hipStream_t st) {
   LAKER_CHECK_ARG(h != nullptr && dst != nullptr, "bad copy_out arguments");
   Precond* pc = reinterpret_cast<Precond*>(h);
+  if (pc->kind == PRECOND_JACOBI) {
+    LAKER_CHECK_ARG(which == 4, "a Jacobi preconditioner holds only its inverse diagonal (which = 4)");
+    LAKER_HIP_CHECK(hipMemcpyAsync(dst, pc->inv_diag.ptr, (size_t)pc->n * (pc->dtype == LAKER_F32 ? 4 : 8),
+                                   hipMemcpyDeviceToDevice, st));
+    return LAKER_OK;
+  }
   const int64_t nn = (int64_t)pc->nr * pc->nr;
   const void* src = nullptr;
   size_t bytes = 0;
@@ -1046,6 +1202,7 @@ int laker_precond_info(laker_precond_t h, double* isotropic_coef, int32_t* nr, d
                        const double** s_matrix) {
   LAKER_CHECK_ARG(h != nullptr, "preconditioner handle is NULL");
   Precond* pc = reinterpret_cast<Precond*>(h);
+  LAKER_CHECK_ARG(pc->kind == PRECOND_CCCP, "not a CCCP preconditioner");
   if (isotropic_coef) *isotropic_coef = pc->a;
   if (nr) *nr = pc->nr;
   if (rho) *rho = pc->rho;

@@ -20,6 +20,8 @@
 
 namespace laker {
 
+int64_t precond_rows(const struct Precond& pc);
+int precond_dtype(const struct Precond& pc);
 int precond_apply(struct Precond& pc, const void* X, int k, int64_t ldx, void* Z, int64_t ldz,
                   hipStream_t st);
 
@@ -556,6 +558,8 @@ int laker_pcg_solve(laker_operator_t oh, laker_precond_t ph, const void* B, int3
   LAKER_CHECK_ARG(B && X && result && k >= 1, "bad pcg arguments");
   Operator& op = *reinterpret_cast<Operator*>(oh);
   Precond* pc = reinterpret_cast<Precond*>(ph);
+  LAKER_CHECK_ARG(!pc || precond_rows(*pc) == op.n, "preconditioner and operator sizes differ");
+  LAKER_CHECK_ARG(!pc || precond_dtype(*pc) == op.dtype, "preconditioner and operator dtypes differ");
   if (op.dtype == LAKER_F64) {
     const double eps = breakdown_eps >= 0 ? breakdown_eps : std::sqrt(2.220446049250313e-16);
     return pcg_run<double>(op, pc, (const double*)B, k, (double*)X, (const double*)X0, tol, max_iter, eps,
@@ -578,6 +582,8 @@ int laker_solve_mixed(laker_operator_t oh, laker_precond_t ph, const double* B, 
   LAKER_CHECK_ARG(inner_precision == LAKER_PREC_MIXED || inner_precision == LAKER_PREC_FP32,
                   "inner precision must be LAKER_PREC_MIXED or LAKER_PREC_FP32");
   Precond* pc = reinterpret_cast<Precond*>(ph);
+  LAKER_CHECK_ARG(!pc || precond_rows(*pc) == op.n, "preconditioner and operator sizes differ");
+  LAKER_CHECK_ARG(!pc || precond_dtype(*pc) == LAKER_F32, "mixed solve needs an FP32 preconditioner");
   const int64_t n = op.n, nk = n * k;
   const size_t vb = round_up(nk * 8, 256);
   LAKER_TRY(op.mixed_ws.ensure(3 * vb));

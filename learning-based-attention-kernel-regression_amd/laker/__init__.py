@@ -17,8 +17,9 @@ from .data import generate_grid, generate_radio_field, pathloss_problem
 from .embeddings import PositionEmbedding
 from .kernels import AttentionKernelOperator, KernelOperator
 from .models import LAKERRegressor
-from .preconditioner import CCCPPreconditioner
-from .solvers import PreconditionedConjugateGradient
+from .preconditioner import AdaptivePreconditioner, CCCPPreconditioner
+from .solvers import JacobiPreconditioner, PreconditionedConjugateGradient
+from .streaming import StreamingUpdater
 
 __version__ = "0.4.0"
 
@@ -32,6 +33,9 @@ __all__ = [
     "AttentionKernelOperator",
     "KernelOperator",
     "CCCPPreconditioner",
+    "AdaptivePreconditioner",
+    "JacobiPreconditioner",
     "PreconditionedConjugateGradient",
+    "StreamingUpdater",
     "LAKERRegressor",
 ]

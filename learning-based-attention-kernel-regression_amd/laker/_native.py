@@ -120,6 +120,11 @@ def _load():
         "laker_coldot": (c_int32, [c_void_p, c_void_p, c_int64, c_int32, c_int64, c_int32, c_void_p,
                                    c_void_p]),
         "laker_set_graphs": (c_int32, [c_int32]),
+        "laker_precond_create_jacobi": (c_int32, [c_void_p, c_int64, c_int32, c_void_p, ctypes.POINTER(c_void_p)]),
+        "laker_precond_from_factors": (c_int32, [c_void_p, c_int64, c_int32, c_int32, c_double, c_double, c_void_p,
+                                                 c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_void_p)]),
+        "laker_orthonormalize": (c_int32, [c_void_p, c_int64, c_int32, c_int32, c_void_p]),
+        "laker_randn": (c_int32, [c_void_p, c_int64, c_int32, ctypes.c_uint64, c_void_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -137,7 +142,8 @@ EXPORTED = [
     "laker_to_dense", "laker_matvec_backward", "laker_cross_matvec_backward", "laker_precond_build",
     "laker_precond_build_from_responses", "laker_precond_copy_out", "laker_precond_destroy",
     "laker_precond_info", "laker_precond_apply", "laker_pcg_solve", "laker_solve_mixed",
-    "laker_set_graphs", "laker_coldot",
+    "laker_set_graphs", "laker_coldot", "laker_precond_create_jacobi", "laker_precond_from_factors",
+    "laker_orthonormalize", "laker_randn",
 ]
 
 

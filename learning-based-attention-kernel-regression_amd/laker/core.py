@@ -20,7 +20,7 @@ from . import _native as N
 from .backend import compute_device, get_default_device, get_default_dtype
 from .embeddings import PositionEmbedding
 from .kernels import AttentionKernelOperator, KernelOperator
-from .preconditioner import CCCPPreconditioner
+from .preconditioner import AdaptivePreconditioner, CCCPPreconditioner
 from .solvers import PreconditionedConjugateGradient, _coldot
 
 logger = logging.getLogger(__name__)
@@ -163,9 +163,18 @@ class LAKERCore:
     def build_preconditioner(self, matvec: Callable[[torch.Tensor], torch.Tensor], n: int,
                              gamma: Optional[float] = None, num_probes: Optional[int] = None,
                              seed: Optional[int] = None, diagonal: Optional[torch.Tensor] = None):
-        """CCCP preconditioner (laker/core.py:461-521)."""
+        """CCCP or adaptive preconditioner (laker/core.py:461-521)."""
+        if self.preconditioner_strategy == "adaptive":
+            pre = AdaptivePreconditioner(
+                gamma=gamma if gamma is not None else self.gamma,
+                num_probes=num_probes if num_probes is not None else self.num_probes, epsilon=self.epsilon,
+                base_rho=self.base_rho, max_iter=self.cccp_max_iter, tol=self.cccp_tol, verbose=self.verbose,
+                device=self.device, dtype=self.dtype, probe_source=self.probe_source,
+                probe_precision=self.probe_precision)
+            return pre.build(matvec, n, diagonal=diagonal, seed=seed)
         if self.preconditioner_strategy != "cccp":
-            raise ValueError("preconditioner_strategy must be 'cccp' in the HIP solver core")
+            raise ValueError(f"preconditioner_strategy must be 'cccp' or 'adaptive', got "
+                             f"{self.preconditioner_strategy!r}")
         pre = CCCPPreconditioner(num_probes=num_probes if num_probes is not None else self.num_probes,
                                  gamma=gamma if gamma is not None else self.gamma, epsilon=self.epsilon,
                                  base_rho=self.base_rho, max_iter=self.cccp_max_iter, tol=self.cccp_tol,

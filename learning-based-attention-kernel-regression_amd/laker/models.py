@@ -3,8 +3,9 @@
 ``LAKERRegressor`` keeps the reference constructor (validation, defaults),
 hyperparameter delegation to the core (``HYPERPARAMS``), ``fit``,
 ``predict``, ``predict_variance``, ``score`` (negative RMSE), ``score_r2``,
-``get_params`` / ``set_params``, ``save`` / ``load`` and
-``fit_learned_embeddings``.  Added keyword arguments: ``precision``,
+``get_params`` / ``set_params``, ``save`` / ``load``,
+``fit_learned_embeddings`` and the warm-started ``partial_fit`` /
+``fit_path`` / ``fit_continuation``.  Added keyword arguments: ``precision``,
 ``probe_source``, ``probe_precision``, ``inner_tol``.
 """
 
@@ -18,6 +19,7 @@ import torch.nn as nn
 
 from .backend import to_tensor
 from .core import LAKERCore
+from .streaming import StreamingUpdater
 
 logger = logging.getLogger(__name__)
 
@@ -110,6 +112,7 @@ class LAKERRegressor:
         self.y_train: Optional[torch.Tensor] = None
         self.partial_fit_count: int = 0
         self.pcg_iterations_: int = 0
+        self.streaming = StreamingUpdater(self.core)
 
     # hyperparameter delegation (laker/models.py:343-387)
     def __getattr__(self, name: str) -> Any:
@@ -141,7 +144,8 @@ class LAKERRegressor:
         self.embeddings, self.embedding_model = self.core.compute_embeddings(x)
         op = self.core.build_kernel_operator(self.embeddings)
         self.kernel_operator = op
-        self.preconditioner = self.core.build_preconditioner(op.matvec, self.embeddings.shape[0], seed=seed)
+        self.preconditioner = self.core.build_preconditioner(op.matvec, self.embeddings.shape[0], seed=seed,
+                                                             diagonal=op.diagonal())
         self.alpha, self.pcg_iterations_ = self.core.solve_pcg(op, self.preconditioner, y, x0=x0)
         solver = getattr(self.core, "last_solver", None)
         self.alpha_fp64 = solver.solution_fp64 if solver is not None else None
@@ -166,6 +170,10 @@ class LAKERRegressor:
         x = to_tensor(x, device=self.device, dtype=self.dtype)
         if x.dim() != 2:
             raise ValueError(f"x must be 2-D, got shape {x.shape}")
+        if self.preconditioner is None:  # loaded from a file without stored factors
+            op = self.kernel_operator
+            self.preconditioner = self.core.build_preconditioner(op.matvec, self.embeddings.shape[0],
+                                                                 diagonal=op.diagonal())
         return self.core.predict_variance(x, self.embedding_model, self.embeddings, self.kernel_operator,
                                           self.preconditioner, self.alpha, self.lambda_reg)
 
@@ -214,14 +222,26 @@ class LAKERRegressor:
     def fit_with_bo(self, *a, **k):
         self._unavailable("fit_with_bo")
 
-    def partial_fit(self, *a, **k):
-        self._unavailable("partial_fit")
+    # streaming (laker/models.py:595-636)
+    def partial_fit(self, x_new, y_new, forgetting_factor: float = 1.0,
+                    rebuild_threshold: int = 100) -> "LAKERRegressor":
+        """Append observations and re-solve warm-started (laker/streaming.py:75-187)."""
+        x_new = to_tensor(x_new, device=self.device, dtype=self.dtype)
+        y_new = to_tensor(y_new, device=self.device, dtype=self.dtype).squeeze()
+        return self.streaming.partial_fit(self, x_new, y_new, forgetting_factor, rebuild_threshold)
 
-    def fit_path(self, *a, **k):
-        self._unavailable("fit_path")
+    def fit_path(self, x, y, lambda_reg_grid, reuse_precond: bool = True) -> dict:
+        """Warm-started solves over a lambda grid (laker/streaming.py:189-304)."""
+        x = to_tensor(x, device=self.device, dtype=self.dtype)
+        y = to_tensor(y, device=self.device, dtype=self.dtype).squeeze()
+        return self.streaming.fit_path(self, x, y, lambda_reg_grid, reuse_precond)
 
-    def fit_continuation(self, *a, **k):
-        self._unavailable("fit_continuation")
+    def fit_continuation(self, x, y, lambda_max: Optional[float] = None, lambda_min: Optional[float] = None,
+                         n_stages: int = 5, reuse_precond: bool = True) -> "LAKERRegressor":
+        """Geometric lambda continuation (laker/streaming.py:306-379)."""
+        x = to_tensor(x, device=self.device, dtype=self.dtype)
+        y = to_tensor(y, device=self.device, dtype=self.dtype).squeeze()
+        return self.streaming.fit_continuation(self, x, y, lambda_max, lambda_min, n_stages, reuse_precond)
 
     def fit_residual_corrector(self, *a, **k):
         self._unavailable("fit_residual_corrector")

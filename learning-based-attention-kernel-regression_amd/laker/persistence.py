@@ -2,9 +2,14 @@
 
 The dict carries the same hyperparameter keys, ``embeddings``, ``alpha`` and
 the embedding-module state dict, so files written by either implementation
-hold the same fields.  Loading uses ``torch.load(weights_only=True)`` (no
-code from the file runs) and rebuilds the exact operator; the preconditioner
-is rebuilt on demand, as in the reference (which does not store it).
+hold the same fields.  It is extended compatibly with the CCCP factors
+(``preconditioner_*`` keys: Q, the QR factor R, C, S = M^{-1/2} - a^{-1/2} I,
+isotropic_coef and rho), which the reference's format lacks: a model saved
+here reloads with its preconditioner handle rebuilt from those factors, with
+no probe matvec or CCCP iteration, so a served system is ready to solve at
+once.  Files without those keys (the reference's) load as before and build
+the preconditioner on demand.  Loading uses ``torch.load(weights_only=True)``
+(no code from the file runs) and rebuilds the exact operator.
 """
 
 from __future__ import annotations
@@ -28,6 +33,16 @@ def save_model(reg, path: str) -> None:
     })
     if reg.alpha_fp64 is not None:
         state["alpha_fp64"] = reg.alpha_fp64.detach().cpu()
+    pre = _cccp_of(reg.preconditioner)
+    if pre is not None:
+        state.update({
+            "preconditioner_q_basis": pre.q_basis.detach().cpu(),
+            "preconditioner_qr_r": pre.qr_r.detach().cpu(),
+            "preconditioner_q_basis_matrix": pre.q_basis_matrix.detach().cpu(),
+            "preconditioner_s_matrix": pre.s_matrix.detach().cpu(),
+            "preconditioner_isotropic_coef": float(pre.isotropic_coef),
+            "preconditioner_rho": float(pre.rho if pre.rho is not None else 0.0),
+        })
     if reg.embedding_model is not None:
         state["embedding_model_state"] = {k: v.detach().cpu() for k, v in reg.embedding_model.state_dict().items()}
         state["embedding_model_class"] = reg.embedding_model.__class__.__name__
@@ -59,4 +74,23 @@ def load_model(path: str):
             model.load_state_dict(state["embedding_model_state"])
             reg.embedding_model = model
     reg.kernel_operator = reg.core.build_kernel_operator(reg.embeddings)
+    if "preconditioner_q_basis" in state:
+        from .preconditioner import CCCPPreconditioner
+
+        reg.preconditioner = CCCPPreconditioner.from_factors(
+            state["preconditioner_q_basis"], state["preconditioner_qr_r"], state["preconditioner_q_basis_matrix"],
+            state["preconditioner_s_matrix"], state["preconditioner_isotropic_coef"],
+            state.get("preconditioner_rho", 0.0), gamma=reg.gamma, epsilon=reg.epsilon, base_rho=reg.base_rho,
+            max_iter=reg.cccp_max_iter, tol=reg.cccp_tol)
     return reg
+
+
+def _cccp_of(pre):
+    """The built CCCP preconditioner behind ``pre`` (itself or an adaptive wrapper's), else None."""
+    from .preconditioner import AdaptivePreconditioner, CCCPPreconditioner
+
+    if isinstance(pre, AdaptivePreconditioner):
+        pre = pre.inner
+    if isinstance(pre, CCCPPreconditioner) and pre.built:
+        return pre
+    return None

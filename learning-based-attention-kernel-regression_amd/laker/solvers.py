@@ -44,19 +44,85 @@ def _native_operator(operator):
 
 
 def _native_preconditioner(preconditioner):
-    from .preconditioner import CCCPPreconditioner
+    """``(handle owner, True)`` when ``preconditioner`` is a native CCCP/Jacobi
+    apply (or an adaptive wrapper around one), ``(None, True)`` for None and
+    ``(None, False)`` for any other callable."""
+    from .preconditioner import AdaptivePreconditioner, CCCPPreconditioner
 
     if preconditioner is None:
         return None, True
     obj = getattr(preconditioner, "__self__", None)
-    if isinstance(obj, CCCPPreconditioner) and getattr(preconditioner, "__func__", None) in (
+    fn = getattr(preconditioner, "__func__", None)
+    if isinstance(obj, CCCPPreconditioner) and fn in (
         CCCPPreconditioner.apply, CCCPPreconditioner.apply_1d, CCCPPreconditioner.apply_2d,
         CCCPPreconditioner.apply_impl,
     ):
         return obj, True
-    if isinstance(preconditioner, CCCPPreconditioner):
+    if isinstance(obj, JacobiPreconditioner) and fn is JacobiPreconditioner.apply:
+        return obj, True
+    if isinstance(obj, AdaptivePreconditioner) and fn is AdaptivePreconditioner.apply and obj.inner is not None:
+        return _native_preconditioner(obj.inner.apply)
+    if isinstance(preconditioner, (CCCPPreconditioner, JacobiPreconditioner)):
         return preconditioner, True
     return None, False
+
+
+class JacobiPreconditioner:
+    """Diagonal preconditioner ``P = diag(lambda I + G)^{-1}`` (reference
+    laker/solvers.py:505-546), held and applied by the HIP library.
+
+    ``inv_diag = 1 / diagonal.clamp(min=eps)`` is computed on the GPU in the
+    diagonal's dtype (float32 or float64; other dtypes are taken as float32);
+    ``apply`` scales every row of ``(n,)`` or ``(n, k)`` input by it.  For a
+    2-D input the reference's ``inv_diag * x`` broadcasts over the last axis
+    (it raises unless k == n); this build applies the documented row scaling.
+    Bound ``apply`` methods drive the device PCG loop like the CCCP apply.
+    """
+
+    def __init__(self, diagonal: torch.Tensor) -> None:
+        from .preconditioner import _PrecondHandle
+
+        if diagonal.dim() != 1:
+            raise ValueError(f"diagonal must be 1-D, got shape {diagonal.shape}")
+        dtype = diagonal.dtype if diagonal.dtype in (torch.float32, torch.float64) else torch.float32
+        d = diagonal.detach().to(device=compute_device(), dtype=dtype).contiguous()
+        self.n = d.shape[0]
+        self.dtype = dtype
+        self._h = _PrecondHandle()
+        N.check(N.lib.laker_precond_create_jacobi(N.ptr(d), self.n, N.dtype_code(dtype), N.stream_ptr(),
+                                                  ctypes.byref(self._h.h)))
+        self._inv = None
+
+    @property
+    def handle(self):
+        return self._h.h
+
+    @property
+    def inv_diag(self) -> torch.Tensor:
+        """``1 / max(diag, eps)``, shape ``(n,)``."""
+        if self._inv is None:
+            t = torch.empty(self.n, dtype=self.dtype, device=compute_device())
+            N.check(N.lib.laker_precond_copy_out(self._h.h, 4, N.ptr(t), N.stream_ptr()))
+            self._inv = t
+        return self._inv
+
+    def apply(self, x: torch.Tensor) -> torch.Tensor:
+        """``P x`` for ``(n,)`` or ``(n, k)`` input."""
+        if x.dim() not in (1, 2) or x.shape[0] != self.n:
+            raise ValueError(f"x must be ({self.n},) or ({self.n}, k), got shape {tuple(x.shape)}")
+        out_dev = x.device
+        xd = x.to(device=compute_device(), dtype=self.dtype).contiguous()
+        x2 = xd.unsqueeze(1) if xd.dim() == 1 else xd
+        z = torch.empty_like(x2)
+        N.check(N.lib.laker_precond_apply(self._h.h, N.ptr(x2), x2.shape[1], x2.shape[1], N.ptr(z), z.shape[1],
+                                          N.stream_ptr()))
+        z = z.squeeze(1) if x.dim() == 1 else z
+        return z.to(out_dev)
+
+    __call__ = apply
+
+    def close(self) -> None:
+        self._h.close()
 
 
 def _coldot(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
@@ -109,6 +175,8 @@ class PreconditionedConjugateGradient:
             raise ValueError(f"rhs must be 1-D or 2-D, got shape {rhs.shape}")
         op = _native_operator(operator)
         pc, pc_ok = _native_preconditioner(preconditioner)
+        if op is not None and pc is not None and (pc.dtype != op.dtype or pc.n != op.n):
+            pc_ok = False  # the device loop needs a preconditioner of the operator's size and dtype
         if op is not None and pc_ok and not self.autograd_safe and not (
             torch.is_grad_enabled() and (rhs.requires_grad or (x0 is not None and x0.requires_grad))
         ):

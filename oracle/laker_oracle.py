@@ -30,6 +30,19 @@ Pinning (what checks each function independently of this restatement):
   module by ``tests/golden/make_golden.py`` (exact float equality).
 * ``dense_solve_fp64``: the paper's n=3 worked example
   (reference tests/test_models.py:37-88) and residual checks.
+* ``jacobi_inverse`` / ``jacobi_apply``: closed forms on diagonal operators
+  (the exact preconditioner of a diagonal system: one PCG step), the eps
+  clamp, reference tests/test_solvers.py:59-73.
+* ``adaptive_select``: diagonal operators of known spectrum (the power
+  iterate tends to lambda_max, Rayleigh quotients bound lambda_min from
+  above) and one operator per selection regime.
+* ``power_iter_probes``: orthonormality of the power block, its span equal
+  to span(A^s R_block) by projection, untouched Gaussian columns, and the
+  reference's lambda_max(P A) comparison (tests/test_preconditioner.py:86-150).
+* ``continuation_schedule`` / ``fit_path`` / ``partial_fit``: the closed-form
+  geometric schedule, each path solution against the FP64 dense solve of its
+  own lambda, and the warm-start identity (a solve started at the exact
+  solution takes no step).
 """
 
 from __future__ import annotations
@@ -513,3 +526,126 @@ def cross_matvec_fp64(query, embeddings, x, scale: float = 1.0, clamp: Optional[
             np.minimum(g, clamp, out=g)
         out[i0:i0 + block] = np.exp(g) @ xv
     return out
+
+
+# ---------------------------------------------------------------------------
+# Jacobi / adaptive preconditioners, power-iteration probes, warm-started refits
+# (reference laker/solvers.py:505-546, laker/preconditioner.py:174-184,382-599,
+# laker/streaming.py:75-379)
+# ---------------------------------------------------------------------------
+
+
+def jacobi_inverse(diagonal: torch.Tensor) -> torch.Tensor:
+    """``1 / diagonal.clamp(min=eps(dtype))`` -- laker/solvers.py:521-532."""
+    return 1.0 / diagonal.clamp(min=torch.finfo(diagonal.dtype).eps)
+
+
+def jacobi_apply(inv_diag: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
+    """``inv_diag * x`` -- laker/solvers.py:534-546 for (n,) input.  For (n, k)
+    input the rows are scaled (the documented intent; the reference's
+    broadcast runs over the last axis and raises unless k == n)."""
+    return inv_diag * x if x.dim() == 1 else inv_diag.unsqueeze(1) * x
+
+
+def adaptive_select(operator: Callable[[torch.Tensor], torch.Tensor], probes: torch.Tensor,
+                    have_diagonal: bool) -> dict:
+    """Conditioning estimate and choice of AdaptivePreconditioner.build
+    (laker/preconditioner.py:469-574) on the given ``min(10, n)`` probes: one
+    operator application to all probes, 5 normalised power steps from the first
+    response, ``lambda_max = v^T A v``, ``lambda_min`` = smallest probe Rayleigh
+    quotient, ``kappa = lambda_max / max(lambda_min, 1e-12)``; Jacobi below 1e3
+    (when a diagonal is given), CCCP below 1e6, else CCCP with doubled probes."""
+    op_probes = operator(probes)
+    v = op_probes[:, 0].clone()
+    for _ in range(5):
+        v = operator(v.unsqueeze(1)).squeeze(1)
+        v = v / torch.linalg.norm(v)
+    lam_max = torch.dot(v, operator(v)).item()
+    rayleigh = torch.sum(probes * op_probes, dim=0) / torch.sum(probes ** 2, dim=0)
+    lam_min = rayleigh.min().item()
+    cond = lam_max / max(lam_min, 1e-12)
+    if cond < 1e3 and have_diagonal:
+        choice = "jacobi"
+    elif cond < 1e6:
+        choice = "cccp"
+    else:
+        choice = "cccp_aggressive"
+    return {"lambda_max": lam_max, "lambda_min": lam_min, "cond": cond, "choice": choice}
+
+
+def power_iter_probes(operator: Callable[[torch.Tensor], torch.Tensor], probes: torch.Tensor,
+                      steps: int = 3) -> torch.Tensor:
+    """The "power_iter" probe strategy (laker/preconditioner.py:174-184): the
+    first ``max(1, int(0.25 N_r))`` columns take ``steps`` rounds of
+    ``Q(A block)`` (reduced Householder QR), the rest stay as drawn."""
+    nr = probes.shape[1]
+    n_power = max(1, int(nr * 0.25))
+    block = probes[:, :n_power]
+    for _ in range(steps):
+        block = operator(block)
+        block = torch.linalg.qr(block, mode="reduced")[0]
+    return torch.cat([block, probes[:, n_power:]], dim=1)
+
+
+def continuation_schedule(lambda_max: float, lambda_min: float, n_stages: int) -> list:
+    """``lambda_k = lambda_max r^k``, ``r = (lambda_min/lambda_max)^(1/(n-1))``,
+    last entry set to lambda_min -- laker/streaming.py:351-373."""
+    if n_stages < 1:
+        raise ValueError(f"n_stages must be positive, got {n_stages}")
+    if lambda_max <= 0 or lambda_min <= 0:
+        raise ValueError("lambda_max and lambda_min must be positive")
+    ratio = (lambda_min / lambda_max) ** (1.0 / max(1, n_stages - 1))
+    sched = [lambda_max * ratio ** k for k in range(n_stages)]
+    sched[-1] = lambda_min
+    return sched
+
+
+def fit_path(embeddings: torch.Tensor, y: torch.Tensor, lambda_grid, probes: torch.Tensor,
+             reuse_precond: bool = True, gamma: float = 0.1, epsilon: float = 1e-8, base_rho: float = 0.05,
+             cccp_max_iter: int = 200, cccp_tol: float = 1e-6, pcg_tol: float = 1e-6,
+             pcg_max_iter: int = 1000) -> dict:
+    """StreamingUpdater.fit_path on fixed embeddings (laker/streaming.py:189-304):
+    lambdas sorted descending, each PCG warm-started from the previous alpha,
+    the CCCP factors of the first stage reused when ``reuse_precond``, final
+    relative residuals ``||A alpha - y|| / ||y||`` in the solver dtype.  The
+    same ``probes`` feed every CCCP build (the reference draws them from the
+    same default-seeded generator each time)."""
+    lambdas = sorted(lambda_grid, reverse=True)
+    n = embeddings.shape[0]
+    skip = skip_clamp_flag(embeddings)
+    alphas, iters, rel = [], [], []
+    x0 = None
+    fac = None
+    for lam in lambdas:
+        def op(v, lam=lam):
+            return attention_matvec(embeddings, lam, v, None, skip)
+
+        if fac is None or not reuse_precond:
+            fac = cccp_build(op, n, probes, gamma, epsilon, base_rho, cccp_max_iter, cccp_tol)
+        res = pcg_solve(op, lambda v, f=fac: cccp_apply(v, f), y, x0=x0, tol=pcg_tol, max_iter=pcg_max_iter)
+        alphas.append(res.x)
+        iters.append(res.iterations)
+        rel.append((torch.linalg.norm(op(res.x) - y) / torch.linalg.norm(y)).item())
+        x0 = res.x.clone()
+    return {"lambda_reg": lambdas, "alphas": alphas, "pcg_iters": iters, "final_rel_res": rel}
+
+
+def partial_fit(embeddings: torch.Tensor, new_embeddings: torch.Tensor, alpha: torch.Tensor, y_old: torch.Tensor,
+                y_new: torch.Tensor, lambda_reg: float, probes: torch.Tensor, forgetting_factor: float = 1.0,
+                gamma: float = 0.1, epsilon: float = 1e-8, base_rho: float = 0.05, cccp_max_iter: int = 200,
+                cccp_tol: float = 1e-6, pcg_tol: float = 1e-6, pcg_max_iter: int = 1000) -> dict:
+    """StreamingUpdater.partial_fit (laker/streaming.py:75-187) on given
+    embeddings: append, rebuild the operator and the CCCP factors (on
+    ``probes``), and re-solve from ``[forgetting_factor * alpha, 0]``."""
+    e = torch.cat([embeddings, new_embeddings], dim=0)
+    n = e.shape[0]
+    skip = skip_clamp_flag(e)
+
+    def op(v):
+        return attention_matvec(e, lambda_reg, v, None, skip)
+
+    y = torch.cat([y_old, y_new])
+    x0 = torch.cat([alpha * forgetting_factor, torch.zeros(new_embeddings.shape[0], dtype=alpha.dtype)])
+    fac = cccp_build(op, n, probes, gamma, epsilon, base_rho, cccp_max_iter, cccp_tol)
+    res = pcg_solve(op, lambda v: cccp_apply(v, fac), y, x0=x0, tol=pcg_tol, max_iter=pcg_max_iter)
+    return {"embeddings": e, "alpha": res.x, "iterations": res.iterations, "y": y}
