@@ -85,7 +85,7 @@ def test_variance_tiny_exact(cuda):
     eq = torch.randn(1, 2, dtype=torch.float64, generator=gen)
     a = O.dense_matrix_fp64(e.numpy(), 0.1)
     kq = np.exp(eq.numpy() @ e.numpy().T)
-    exact = float(np.exp(eq.numpy() @ eq.numpy().T)[0, 0] - kq @ np.linalg.solve(a, kq.T))
+    exact = float((np.exp(eq.numpy() @ eq.numpy().T)[0, 0] - kq @ np.linalg.solve(a, kq.T)).item())
 
     class Fixed(torch.nn.Module):
         def __init__(self, t):
