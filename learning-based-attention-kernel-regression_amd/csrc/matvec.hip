@@ -834,17 +834,19 @@ __global__ __launch_bounds__(256) void mv_multi_kernel(
 // rounded to bf16; GX = LAKER_PREC_MIXED: both split hi/lo, 3 products).
 // X is packed beforehand (pack_x_kernel) straight into the MFMA B-operand
 // layout, so a lane fetches each operand with one 16-byte load; the exp tile
-// P is converted once per row tile and reused for all 64 columns of a pass.
-// Xp index: ((t * 2 + q) * 2 + u) * 64 + lane, 8 bf16 values
+// P is converted once per row tile and reused for all 32 nq columns of a pass
+// (nq = 1, 2 or 4 column chunks of 32).
+// Xp index: ((t * nq + q) * 2 + u) * 64 + lane, 8 bf16 values
 //   X[32 t + jj(8u + v, lane/32)][c_base + 32 q + lane%32], v = 0..7.
 // ---------------------------------------------------------------------------
 __global__ void pack_x_kernel(const float* __restrict__ X, int64_t ldx, int64_t n_cols, int kcols,
-                              int c_base, int64_t ntiles, bf16x8* __restrict__ xh, bf16x8* __restrict__ xl,
-                              bf16x8* __restrict__ xm) {
+                              int c_base, int64_t ntiles, int nq, bf16x8* __restrict__ xh,
+                              bf16x8* __restrict__ xl, bf16x8* __restrict__ xm) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= ntiles * 256) return;
-  const int lane = (int)(idx & 63), u = (int)((idx >> 6) & 1), q = (int)((idx >> 7) & 1);
-  const int64_t t = idx >> 8;
+  if (idx >= ntiles * 128 * nq) return;
+  const int lane = (int)(idx & 63), u = (int)((idx >> 6) & 1);
+  const int q = (int)((idx >> 7) % nq);
+  const int64_t t = (idx >> 7) / nq;
   const int h = lane >> 5, li = lane & 31;
   const int c = c_base + 32 * q + li;
   // x = hi + lo (hi/lo mode) or x = hi + mid + lo3 (xm given: 3-way split)
@@ -923,7 +925,7 @@ __global__ __launch_bounds__(256) void mv_multi2_kernel(
         const int xi = vi - VA;
         const int tt = xi / XT, rem = xi % XT;
         const int pp = rem / (NCC * 2 * 64), off = rem % (NCC * 2 * 64);  // off = (q*2+u)*64 + lane
-        if (tt < nt) v = xsrc[pp][(t + tt) * 256 + off];
+        if (tt < nt) v = xsrc[pp][(t + tt) * (NCC * 128) + off];
       }
       pre[u2] = v;
     }
@@ -1566,24 +1568,29 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
   const int ns16 = op.ka16 / 16;
   const bool packed_x = mixed && !deriv && gx != LAKER_PREC_FP32 && (ns16 == 2 || ns16 == 4 || ns16 == 6);
   if (packed_x) {
+    // X parts packed per pass: 3 x 2 chunks (3-way split) or 2 x 4 chunks (hi/lo, BF16)
     const int64_t ntiles = ceil_div(n_cols, 32);
-    const size_t third = (size_t)ntiles * 256 * sizeof(bf16x8);
-    LAKER_TRY(op.xpack.ensure(3 * third));
+    const size_t part_bytes = (size_t)ntiles * (gx == GX_SPLIT3 ? 256 : 512) * sizeof(bf16x8);
+    LAKER_TRY(op.xpack.ensure(3 * part_bytes));
     bf16x8* xh = op.xpack.as<bf16x8>();
-    bf16x8* xl = reinterpret_cast<bf16x8*>(op.xpack.as<char>() + third);
-    bf16x8* xm = reinterpret_cast<bf16x8*>(op.xpack.as<char>() + 2 * third);
+    bf16x8* xl = reinterpret_cast<bf16x8*>(op.xpack.as<char>() + part_bytes);
+    bf16x8* xm = reinterpret_cast<bf16x8*>(op.xpack.as<char>() + 2 * part_bytes);
     const int jsplit2 = jsplit;
     const dim3 block(256);
-    for (int c0 = 0; c0 < k; c0 += 64) {
-      hipLaunchKernelGGL(pack_x_kernel, dim3((unsigned)ceil_div(ntiles * 256, 256)), dim3(256), 0, st, X,
-                         ldx, n_cols, k, c0, ntiles, xh, gx == LAKER_PREC_BF16 ? nullptr : xl,
+    for (int c0 = 0; c0 < k;) {
+      // column chunks of 32 per pass: the 3-way split keeps 2 for its register budget;
+      // hi/lo and BF16 take 4 when more than 64 columns remain (half the Gram passes)
+      const int rem = k - c0;
+      const int nq = (gx != GX_SPLIT3 && rem > 64) ? 4 : (rem > 32 ? 2 : 1);
+      hipLaunchKernelGGL(pack_x_kernel, dim3((unsigned)ceil_div(ntiles * 128 * nq, 256)), dim3(256), 0, st, X,
+                         ldx, n_cols, k, c0, ntiles, nq, xh, gx == LAKER_PREC_BF16 ? nullptr : xl,
                          gx == GX_SPLIT3 ? xm : nullptr);
       LAKER_KCHECK("pack_x_kernel");
 #define LAKER_M2J(NS, GXM, NC, JTV)                                                                   \
   do {                                                                                                \
     constexpr size_t lds2 = 2 * (size_t)m2_chunk_bytes(NS, M2Parts<GXM>::value, NC, JTV);            \
-    /* one row tile per wave for the register-heavy 3-way split: higher occupancy */                 \
-    constexpr int ITWV = (GXM == GX_SPLIT3) ? 1 : 2;                                                  \
+    /* one row tile per wave for the register-heavy 3-way split and 4-chunk passes */                \
+    constexpr int ITWV = (GXM == GX_SPLIT3 || NC == 4) ? 1 : 2;                                       \
     dim3 grid2((unsigned)(ceil_div(n_rows, 128 * ITWV) * jsplit2));                                   \
     if (clamp)                                                                                        \
       hipLaunchKernelGGL((mv_multi2_kernel<NS, GXM, NC, JTV, ITWV, true>), grid2, block, lds2, st,     \
@@ -1601,7 +1608,13 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
   } while (0)
 #define LAKER_M2(NS, GXM)                              \
   do {                                                 \
-    if (k - c0 > 32) LAKER_M2C(NS, GXM, 2);            \
+    if (nq == 4) LAKER_M2C(NS, GXM, 4);                \
+    else if (nq == 2) LAKER_M2C(NS, GXM, 2);           \
+    else LAKER_M2C(NS, GXM, 1);                        \
+  } while (0)
+#define LAKER_M2_2(NS, GXM)                            \
+  do {                                                 \
+    if (nq == 2) LAKER_M2C(NS, GXM, 2);                \
     else LAKER_M2C(NS, GXM, 1);                        \
   } while (0)
 #define LAKER_M2_NS(GXM)                     \
@@ -1610,14 +1623,23 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
     else if (ns16 == 4) LAKER_M2(4, GXM);    \
     else LAKER_M2(6, GXM);                   \
   } while (0)
-      if (gx == GX_SPLIT3) LAKER_M2_NS(GX_SPLIT3);
+#define LAKER_M2_NS3(GXM)                    \
+  do {                                       \
+    if (ns16 == 2) LAKER_M2_2(2, GXM);       \
+    else if (ns16 == 4) LAKER_M2_2(4, GXM);  \
+    else LAKER_M2_2(6, GXM);                 \
+  } while (0)
+      if (gx == GX_SPLIT3) LAKER_M2_NS3(GX_SPLIT3);
       else if (gx == LAKER_PREC_MIXED) LAKER_M2_NS(LAKER_PREC_MIXED);
       else LAKER_M2_NS(LAKER_PREC_BF16);
 #undef LAKER_M2_NS
+#undef LAKER_M2_NS3
 #undef LAKER_M2
+#undef LAKER_M2_2
 #undef LAKER_M2C
 #undef LAKER_M2J
       LAKER_KCHECK("mv_multi2_kernel");
+      c0 += 32 * nq;
     }
   } else {
     for (int c0 = 0; c0 < k; c0 += 64) {

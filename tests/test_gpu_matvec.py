@@ -40,7 +40,7 @@ def make(n, d, k, seed, amp=0.6, dtype=np.float32):
 
 CASES = [(1, 1, 1), (5, 3, 1), (31, 10, 1), (64, 10, 2), (100, 6, 3), (257, 16, 4), (300, 10, 5),
          (513, 16, 31), (700, 10, 64), (1000, 32, 70), (2000, 10, 1), (2000, 16, 1), (5000, 10, 1),
-         (9000, 16, 1)]
+         (9000, 16, 1), (600, 4, 97), (800, 16, 130), (1500, 10, 200)]
 
 
 @pytest.mark.parametrize("n,d,k", CASES)
