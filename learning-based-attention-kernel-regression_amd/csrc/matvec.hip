@@ -461,18 +461,30 @@ __global__ __launch_bounds__(256) void mv_vec1_kernel(
 
 // ---------------------------------------------------------------------------
 // Symmetric k = 1 product of the square operator.  Workgroup (b, seg) owns
-// rows [256 b, 256 b + 256) and the column tiles of segment seg,
-// [SYM_SEG seg, SYM_SEG (seg + 1)), at or right of tile 8 b: only tiles at or
-// right of the diagonal 256 x 256 block are visited, so every exponential is evaluated once for the pair (i, j), (j, i).
+// rows [SYM_RB b, SYM_RB (b + 1)) (4 waves x SYM_ITW row tiles) and the column
+// tiles of segment seg, [SYM_SEG seg, SYM_SEG (seg + 1)), at or right of tile
+// SYM_DT b: only tiles at or right of the diagonal SYM_RB x SYM_RB block are
+// visited, so every exponential is evaluated once for the pair (i, j), (j, i).
 // Each exp tile P feeds the row sums y_I += P x_J (in-lane, as in the
 // non-symmetric kernel) and, right of the diagonal block, the column sums
-// y_J += P^T x_I: a 32-lane transpose-reduce (one permlane16 swap and four
-// DPP levels), a fixed-order sum over the 4 waves in LDS, and one FP64
-// partial per (row block, column).  Row partials: part_r[seg][i];
-// column partials: part_c[coff(b) + j - 256 (b + 1)] for j >= 256 (b + 1),
-// coff(b) = b n - 128 b (b + 1).
+// y_J += P^T x_I: the products of the wave's row tiles are summed in-lane,
+// then a 32-lane transpose-reduce (one permlane16 swap and four DPP levels)
+// runs once per column tile for all of them, a fixed-order sum over the 4
+// waves in LDS follows, and one FP64 partial is stored per (row block,
+// column).  Row partials: part_r[seg][i]; column partials:
+// part_c[coff(b) + j - SYM_RB (b + 1)] for j >= SYM_RB (b + 1),
+// coff(b) = b n - (SYM_RB / 2) b (b + 1).
+// The kernel is bound by VALU issue (v_exp_f32 takes two FMA slots): per
+// pair of entries one exp, one row FMA and one column FMA; the transpose-
+// reduce (about 40 instructions per column tile) is what SYM_ITW amortises.
 // ---------------------------------------------------------------------------
-constexpr int SYM_SEG = 64;  // column tiles per workgroup
+#ifndef LAKER_SYM_ITW
+#define LAKER_SYM_ITW 4
+#endif
+constexpr int SYM_SEG = 64;                // column tiles per workgroup
+constexpr int SYM_ITW = LAKER_SYM_ITW;     // row tiles per wave
+constexpr int SYM_RB = 4 * 32 * SYM_ITW;   // rows per workgroup
+constexpr int SYM_DT = SYM_RB / 32;        // column tiles of the diagonal block
 
 template <int CTRL>
 __device__ inline float dpp_f(float v) {
@@ -512,7 +524,7 @@ __device__ inline float transpose_reduce16(const float (&q)[16], int lane) {
   return w + dpp_f<0xB1>(w);                 // quad_perm [1,0,3,2]: lane ^ 1
 }
 
-__host__ __device__ inline int64_t sym_coff(int64_t b, int64_t n) { return b * n - 128 * b * (b + 1); }
+__host__ __device__ inline int64_t sym_coff(int64_t b, int64_t n) { return b * n - (SYM_RB / 2) * b * (b + 1); }
 
 template <int MODE, int NS, bool CLAMP>
 __global__ __launch_bounds__(256) void mv_sym1_kernel(
@@ -520,15 +532,15 @@ __global__ __launch_bounds__(256) void mv_sym1_kernel(
     const float* __restrict__ X, int64_t ldx, const int2* __restrict__ map, double* __restrict__ part_r,
     double* __restrict__ part_c, float clampv, const int* __restrict__ skip) {
   using G = GramOps<MODE>;
-  constexpr int ITW = 2;
+  constexpr int ITW = SYM_ITW;
   if (skip && *skip) return;  // uniform across the grid
   extern __shared__ __align__(16) char sym_smem[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, h = lane >> 5, li = lane & 31;
   const int2 bs = map[blockIdx.x];
   const int64_t b = bs.x;
-  const int64_t I0 = b * 256 + w * (32 * ITW);
+  const int64_t I0 = b * SYM_RB + w * (32 * ITW);
   const bool active = I0 < n;
-  const int64_t wrem = (n - b * 256 + 63) / 64;
+  const int64_t wrem = (n - b * SYM_RB + 32 * ITW - 1) / (32 * ITW);
   const int nact = wrem < 4 ? (int)wrem : 4;
   constexpr int gb = MODE == LAKER_PREC_FP32 ? 8 * NS : 32 * NS;
   constexpr int lb = MODE == LAKER_PREC_FP32 ? ((gb / 4) % 2 == 0 ? gb + 4 : gb) : gb + 16;
@@ -540,18 +552,23 @@ __global__ __launch_bounds__(256) void mv_sym1_kernel(
   typename G::template Operand<NS> B[ITW];
   float xr[ITW];
   if (active) {
+    // packed rows exist up to round_up(n, 64): a row tile starting at or
+    // beyond n (the wave's last tiles in the last row block) stays zero
 #pragma unroll
     for (int it = 0; it < ITW; ++it) {
-      G::load(B[it], rowpack, I0 + it * 32 + li, NS * (MODE == LAKER_PREC_FP32 ? 2 : 16), h);
+      if (I0 + it * 32 < n)
+        G::load(B[it], rowpack, I0 + it * 32 + li, NS * (MODE == LAKER_PREC_FP32 ? 2 : 16), h);
+      else
+        B[it] = typename G::template Operand<NS>{};
       const int64_t i = I0 + it * 32 + li;
       xr[it] = i < n ? X[i * ldx] : 0.f;
     }
   }
   const int64_t ntiles = (n + 31) / 32;
-  const int64_t tdiag = 8 * (b + 1);  // first tile right of the diagonal block
+  const int64_t tdiag = SYM_DT * (b + 1);  // first tile right of the diagonal block
   // segments are aligned to absolute column tiles, so the workgroups of one
   // segment (all row blocks) share its columns in L2
-  const int64_t t0 = (8 * b > (int64_t)bs.y * SYM_SEG) ? 8 * b : (int64_t)bs.y * SYM_SEG;
+  const int64_t t0 = (SYM_DT * b > (int64_t)bs.y * SYM_SEG) ? SYM_DT * b : (int64_t)bs.y * SYM_SEG;
   const int64_t t1 = ((int64_t)bs.y * SYM_SEG + SYM_SEG < ntiles) ? (int64_t)bs.y * SYM_SEG + SYM_SEG : ntiles;
   const char* gsrc = reinterpret_cast<const char*>(colpack);
   constexpr int VPT = MODE == LAKER_PREC_FP32 ? (8 * NS + 31) / 32 : NS;
@@ -632,25 +649,32 @@ __global__ __launch_bounds__(256) void mv_sym1_kernel(
 #pragma unroll
           for (int q = 0; q < 4; ++q) xv[4 * g + q] = f[q];
         }
-        floatx16 D[ITW];
+        // row tiles in pairs: two Gram tiles live at a time
+        float q[16];
 #pragma unroll
-        for (int it = 0; it < ITW; ++it) D[it] = G::template gram_exact<NS>(A, B[it]);
+        for (int p = 0; p < ITW / 2; ++p) {
+          floatx16 D[2];
 #pragma unroll
-        for (int it = 0; it < ITW; ++it) {
+          for (int u = 0; u < 2; ++u) D[u] = G::template gram_exact<NS>(A, B[2 * p + u]);
 #pragma unroll
-          for (int v = 0; v < 16; ++v) D[it][v] = kexp<CLAMP, false>(D[it][v], clampv);
-          float a0 = D[it][0] * xv[0], a1 = D[it][1] * xv[1];
+          for (int u = 0; u < 2; ++u) {
 #pragma unroll
-          for (int v = 2; v < 16; v += 2) {
-            a0 = __builtin_fmaf(D[it][v], xv[v], a0);
-            a1 = __builtin_fmaf(D[it][v + 1], xv[v + 1], a1);
+            for (int v = 0; v < 16; ++v) D[u][v] = kexp<CLAMP, false>(D[u][v], clampv);
+            float a0 = D[u][0] * xv[0], a1 = D[u][1] * xv[1];
+#pragma unroll
+            for (int v = 2; v < 16; v += 2) {
+              a0 = __builtin_fmaf(D[u][v], xv[v], a0);
+              a1 = __builtin_fmaf(D[u][v + 1], xv[v + 1], a1);
+            }
+            acc64[2 * p + u] += (double)(a0 + a1);
           }
-          acc64[it] += (double)(a0 + a1);
+          // column-side products of the row tiles, summed in-lane
+#pragma unroll
+          for (int v = 0; v < 16; ++v)
+            q[v] = __builtin_fmaf(D[1][v], xr[2 * p + 1],
+                                  p == 0 ? D[0][v] * xr[0] : __builtin_fmaf(D[0][v], xr[2 * p], q[v]));
         }
         if (tc + tt >= tdiag) {  // uniform across the block
-          float q[16];
-#pragma unroll
-          for (int v = 0; v < 16; ++v) q[v] = __builtin_fmaf(D[1][v], xr[1], D[0][v] * xr[0]);
           const float cs = transpose_reduce16(q, lane);
           if ((lane & 1) == 0) cb[w * (V1_JT * 32) + tt * 32 + jq] = cs;
         }
@@ -666,7 +690,7 @@ __global__ __launch_bounds__(256) void mv_sym1_kernel(
       if (t >= tdiag && j < n) {
         double v = 0.0;
         for (int ww = 0; ww < nact; ++ww) v += (double)cb[ww * (V1_JT * 32) + threadIdx.x];
-        part_c[sym_coff(b, n) + j - 256 * (b + 1)] = v;
+        part_c[sym_coff(b, n) + j - SYM_RB * (b + 1)] = v;
       }
     }
     odd = !odd;
@@ -681,7 +705,7 @@ __global__ __launch_bounds__(256) void mv_sym1_kernel(
   }
 }
 
-// y[i] = lam xr[i] (+ y[i]) + sum_seg part_r[seg][i] + sum_{b < i/256} part_c[coff(b) + i - 256 (b+1)]
+// y[i] = lam xr[i] (+ y[i]) + sum_seg part_r[seg][i] + sum_{b < i/SYM_RB} part_c[coff(b) + i - SYM_RB (b+1)]
 __global__ void mv_sym_reduce_kernel(const double* __restrict__ part_r, const double* __restrict__ part_c,
                                      int64_t n, double lam, const float* __restrict__ Xr, int64_t ldxr,
                                      float* __restrict__ Y, int64_t ldy, int accumulate,
@@ -690,12 +714,12 @@ __global__ void mv_sym_reduce_kernel(const double* __restrict__ part_r, const do
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t ntiles = (n + 31) / 32;
-  const int64_t b = i / 256;
+  const int64_t b = i / SYM_RB;
   const int64_t nseg = (ntiles + SYM_SEG - 1) / SYM_SEG;
   // eight independent chains (fixed assignment) keep the loads in flight
   double a8[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   double tail = 0.0;
-  int64_t sg = (8 * b) / SYM_SEG;  // first segment of row block b
+  int64_t sg = (SYM_DT * b) / SYM_SEG;  // first segment of row block b
   for (; sg + 8 <= nseg; sg += 8) {
 #pragma unroll
     for (int u = 0; u < 8; ++u) a8[u] += part_r[(sg + u) * n + i];
@@ -706,10 +730,10 @@ __global__ void mv_sym_reduce_kernel(const double* __restrict__ part_r, const do
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int64_t q = bb + u;
-      a8[u] += part_c[sym_coff(q, n) + i - 256 * (q + 1)];
+      a8[u] += part_c[sym_coff(q, n) + i - SYM_RB * (q + 1)];
     }
   }
-  for (; bb < b; ++bb) tail += part_c[sym_coff(bb, n) + i - 256 * (bb + 1)];
+  for (; bb < b; ++bb) tail += part_c[sym_coff(bb, n) + i - SYM_RB * (bb + 1)];
   double s = (((a8[0] + a8[1]) + (a8[2] + a8[3])) + ((a8[4] + a8[5]) + (a8[6] + a8[7]))) + tail;
   if (Xr) s += lam * (double)Xr[i * ldxr];
   if (accumulate) s += (double)Y[i * ldy];
@@ -1375,7 +1399,7 @@ static int launch_sym1(Operator& op, int mode, const void* colpack, const void* 
   if (mode == LAKER_PREC_FP32 ? !fp32_ok : !bf16_ok) return 1;
   const int64_t ntiles = ceil_div(n, 32);
   const int64_t nsegmax = ceil_div(ntiles, SYM_SEG);
-  const int64_t ncb = ceil_div(n, 256) - 1;  // row blocks with columns right of their diagonal block
+  const int64_t ncb = ceil_div(n, SYM_RB) - 1;  // row blocks with columns right of their diagonal block
   LAKER_TRY(op.ensure_part((size_t)nsegmax * n * sizeof(double)));
   LAKER_TRY(op.part_c.ensure((size_t)std::max<int64_t>(1, sym_coff(ncb, n)) * sizeof(double)));
   double* part_r = op.part.as<double>();
@@ -1825,9 +1849,9 @@ int laker_operator_create(const void* E, int64_t n, int32_t d, int32_t dtype, do
   {
     // segment-major order: consecutive workgroups (spread over the XCDs)
     // work on the same column segment
-    const int64_t ntiles = ceil_div(n, 32), nb = ceil_div(n, 256), nseg = ceil_div(ntiles, SYM_SEG);
+    const int64_t ntiles = ceil_div(n, 32), nb = ceil_div(n, SYM_RB), nseg = ceil_div(ntiles, SYM_SEG);
     for (int64_t sg = 0; sg < nseg; ++sg)
-      for (int64_t b = 0; b < nb && 8 * b < SYM_SEG * (sg + 1); ++b) smap.push_back(int2{(int)b, (int)sg});
+      for (int64_t b = 0; b < nb && SYM_DT * b < SYM_SEG * (sg + 1); ++b) smap.push_back(int2{(int)b, (int)sg});
   }
   if (e == hipSuccess && (rc = op->sym_map.ensure(smap.size() * sizeof(int2))) != LAKER_OK) {
     delete op;
