@@ -26,10 +26,15 @@ sys.path.insert(0, PKG_ROOT)
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 HBM_PEAK_GBS = 8000.0
-# v_exp_f32 throughput measured on MI355X with a dependency-free kernel
-# (32 exp + 64 FMA vs 64 FMA per wave iteration, 4 waves per SIMD): 11.5 cycles
-# per wave64 instruction, i.e. 64 / 11.5 * 1024 SIMDs * 2.4 GHz
-V_EXP_F32_RATE_T = 13.7
+# VALU issue on MI355X, measured with dependency-free streams of 16 chains per
+# lane, 12 waves per CU: v_fma_f32 4.5 and v_exp_f32 8.6 cycles per wave64
+# instruction at 2.4 GHz, no co-issue of the two (exp + fma: 12.3 cycles).
+# Pure v_exp_f32 stream: 64 lanes * 1024 SIMDs * 2.4e9 / 8.6 = 18.3e12 exp/s.
+V_EXP_F32_RATE_T = 18.3
+FMA_CYCLES = 4.5
+EXP_CYCLES = 8.6
+CLOCK_HZ = 2.4e9
+N_SIMD = 1024
 
 
 def read_traffic(n, d):
@@ -211,6 +216,9 @@ def main():
     t_mv = mv["mixed"] * 1e-3
     traffic = read_traffic(n, d)
     exp_peak = V_EXP_F32_RATE_T
+    # issue-time floor of the symmetric k=1 product: each pair (i, j), (j, i)
+    # costs one exp and two FMAs (row sum with x_j, column sum with x_i)
+    valu_floor_ms = (n * n / 2) / 64 * (EXP_CYCLES + 2 * FMA_CYCLES) / N_SIMD / CLOCK_HZ * 1e3
     roofline = {
         "kernel": "mv_sym1_kernel<LAKER_PREC_MIXED> + mv_sym_reduce_kernel (laker_matvec, k=1, "
                   "BF16-split Gram on MFMA, each exp once per (i,j),(j,i) pair)",
@@ -221,7 +229,12 @@ def main():
         "algorithmic_flops_per_launch": flops_mv,
         "algorithmic_bytes_per_launch": bytes_mv,
         "avg_launch_ms": mv["mixed"],
-        "binding_resource": "v_exp_f32 issue (transcendental VALU), not MFMA or HBM: see exp_rate",
+        "binding_resource": "VALU issue (v_exp_f32 plus the row and column FMAs of every exp), "
+                            "not MFMA or HBM: see valu_floor and exp_rate",
+        "valu_floor": {"ms": valu_floor_ms, "frac": valu_floor_ms / mv["mixed"],
+                       "model": "n^2/2 pairs x (one v_exp_f32 + two v_fma_f32) / 64 lanes at the measured "
+                                f"{EXP_CYCLES} + 2 x {FMA_CYCLES} cycles per wave64 instruction, "
+                                f"{N_SIMD} SIMDs at {CLOCK_HZ / 1e9} GHz"},
         "exp_rate": {"achieved_Texp_s": n * n / t_mv / 1e12, "measured_peak_Texp_s": exp_peak,
                      "frac": n * n / t_mv / 1e12 / exp_peak,
                      "note": "n^2 algorithmic exps per product; the symmetric kernel evaluates n^2/2"},
