@@ -903,7 +903,7 @@ static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, i
                        nr, n_over_nr, epsilon);
     LAKER_KCHECK("cccp_weights_kernel");
     LAKER_TRY((gemm<double, double, double>(false, true, nr, nr, nr, 1.0, Rw.as<double>(), nr, Rq, nr, 0.0,
-                                             Hb.as<double>(), nr, nullptr, st)));
+                                             Hb.as<double>(), nr, &pc->ws, st)));
     hipLaunchKernelGGL(cccp_trace_kernel, dim3(1), dim3(256), 0, st, Hb.as<double>(), nr, gamma, s, rho,
                        a_rho, n, scb.as<CccpScalars>());
     LAKER_KCHECK("cccp_trace_kernel");
@@ -955,7 +955,7 @@ static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, i
   for (int it = 0; it < 200; ++it) {
     // P = Z Y -> Hb ; T -> Rw ; Y <- Y T (Cn) ; Z <- T Z (Linv)
     LAKER_TRY((gemm<double, double, double>(false, false, nr, nr, nr, 1.0, Zb.as<double>(), nr,
-                                             Mb.as<double>(), nr, 0.0, Hb.as<double>(), nr, nullptr, st)));
+                                             Mb.as<double>(), nr, 0.0, Hb.as<double>(), nr, &pc->ws, st)));
     hipLaunchKernelGGL(ns_t_kernel, dim3(ns_blocks), dim3(256), 0, st, Hb.as<double>(), Rw.as<double>(),
                        (int64_t)nr, partb.as<double>());
     LAKER_KCHECK("ns_t_kernel");
@@ -973,9 +973,9 @@ static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, i
     }
     prev_err = err;
     LAKER_TRY((gemm<double, double, double>(false, false, nr, nr, nr, 1.0, Mb.as<double>(), nr,
-                                             Rw.as<double>(), nr, 0.0, Cn.as<double>(), nr, nullptr, st)));
+                                             Rw.as<double>(), nr, 0.0, Cn.as<double>(), nr, &pc->ws, st)));
     LAKER_TRY((gemm<double, double, double>(false, false, nr, nr, nr, 1.0, Rw.as<double>(), nr,
-                                             Zb.as<double>(), nr, 0.0, Linv.as<double>(), nr, nullptr, st)));
+                                             Zb.as<double>(), nr, 0.0, Linv.as<double>(), nr, &pc->ws, st)));
     LAKER_HIP_CHECK(hipMemcpyAsync(Mb.ptr, Cn.ptr, nn * 8, hipMemcpyDeviceToDevice, st));
     LAKER_HIP_CHECK(hipMemcpyAsync(Zb.ptr, Linv.ptr, nn * 8, hipMemcpyDeviceToDevice, st));
   }

@@ -11,12 +11,15 @@
 
 namespace laker {
 
-constexpr int GBM = 64, GBN = 64, GBK = 16;
+constexpr int GBM = 64, GBN = 64, GBK = 32;
+constexpr int GPF = GBK * GBM / 256;  // elements of each operand block per thread
 
 // One 64x64 tile of op(A) op(B) over the K range [k0, k1) on the FP64 MFMA
 // (v_mfma_f64_16x16x4f64).  Operands are converted to double while staged in
-// LDS (k-major); wave w owns the 32x32 quadrant (w / 2, w % 2) as 2x2 MFMA
-// 16x16 blocks.  Lane l supplies A[m = l % 16][k = l / 16], B[k = l / 16][n = l % 16]
+// LDS (k-major), 32 k per block; the next block is loaded into registers while
+// the MFMAs consume the current one, so one global-load latency per block is
+// hidden.  Wave w owns the 32x32 quadrant (w / 2, w % 2) as 2x2 MFMA 16x16
+// blocks.  Lane l supplies A[m = l % 16][k = l / 16], B[k = l / 16][n = l % 16]
 // and receives D[(l / 16) + 4 r][l % 16].
 template <class TA, class TB, bool TRA, bool TRB>
 __device__ inline void gemm_tile(const TA* __restrict__ A, int64_t lda, const TB* __restrict__ B,
@@ -26,25 +29,42 @@ __device__ inline void gemm_tile(const TA* __restrict__ A, int64_t lda, const TB
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6, wm = w >> 1, wn = w & 1;
   const int lm = lane & 15, lk = lane >> 4;
-  for (int64_t kb = k0; kb < k1; kb += GBK) {
-    __syncthreads();
-    for (int e = tid; e < GBK * GBM; e += 256) {
-      int kk, mm;
-      if (TRA) { mm = e % GBM; kk = e / GBM; } else { kk = e % GBK; mm = e / GBK; }
+  // element u of this thread: (kk, mm) of the A block, (kk, nn) of the B block;
+  // consecutive threads take consecutive addresses of the stored layout
+  auto a_idx = [&](int u, int& kk, int& mm) {
+    const int e = tid + 256 * u;
+    if (TRA) { mm = e % GBM; kk = e / GBM; } else { kk = e % GBK; mm = e / GBK; }
+  };
+  auto b_idx = [&](int u, int& kk, int& nn) {
+    const int e = tid + 256 * u;
+    if (TRB) { kk = e % GBK; nn = e / GBK; } else { nn = e % GBN; kk = e / GBN; }
+  };
+  double pa[GPF], pb[GPF];
+  auto load = [&](int64_t kb) {
+#pragma unroll
+    for (int u = 0; u < GPF; ++u) {
+      int kk, mm, nn;
+      a_idx(u, kk, mm);
       const int64_t gm = m0 + mm, gk = kb + kk;
-      double v = 0.0;
-      if (gm < M && gk < k1) v = TRA ? (double)A[gk * lda + gm] : (double)A[gm * lda + gk];
-      As[kk][mm] = v;
+      pa[u] = (gm < M && gk < k1) ? (TRA ? (double)A[gk * lda + gm] : (double)A[gm * lda + gk]) : 0.0;
+      b_idx(u, kk, nn);
+      const int64_t gn = n0 + nn, gk2 = kb + kk;
+      pb[u] = (gn < N && gk2 < k1) ? (TRB ? (double)B[gn * ldb + gk2] : (double)B[gk2 * ldb + gn]) : 0.0;
     }
-    for (int e = tid; e < GBK * GBN; e += 256) {
-      int kk, nn;
-      if (TRB) { kk = e % GBK; nn = e / GBK; } else { nn = e % GBN; kk = e / GBN; }
-      const int64_t gn = n0 + nn, gk = kb + kk;
-      double v = 0.0;
-      if (gn < N && gk < k1) v = TRB ? (double)B[gn * ldb + gk] : (double)B[gk * ldb + gn];
-      Bs[kk][nn] = v;
+  };
+  if (k0 < k1) load(k0);
+  for (int64_t kb = k0; kb < k1; kb += GBK) {
+    __syncthreads();  // the previous block's MFMAs are done with the LDS
+#pragma unroll
+    for (int u = 0; u < GPF; ++u) {
+      int kk, mm, nn;
+      a_idx(u, kk, mm);
+      As[kk][mm] = pa[u];
+      b_idx(u, kk, nn);
+      Bs[kk][nn] = pb[u];
     }
     __syncthreads();
+    if (kb + GBK < k1) load(kb + GBK);
 #pragma unroll
     for (int s4 = 0; s4 < GBK / 4; ++s4) {
       double a[2], b[2];

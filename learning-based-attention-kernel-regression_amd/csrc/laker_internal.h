@@ -44,6 +44,9 @@ struct Operator {
   int sym_nwg = 0;
   bool sym_enabled = true;  // symmetric k = 1 product for the square operator
   DevBuf part_c;    // column partial sums of the symmetric product
+  DevBuf sym64_map; // (row block, segment) of each workgroup of the symmetric FP64 product
+  int sym64_nwg = 0;
+  DevBuf part_c64;  // column partial sums of the symmetric FP64 product
 
   int ensure_part(size_t bytes) { return part.ensure(bytes); }
   bool clamp_for(int policy) const {

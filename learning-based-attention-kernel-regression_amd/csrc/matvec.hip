@@ -1221,6 +1221,176 @@ __global__ __launch_bounds__(256) void mv_f64m_kernel(const double* __restrict__
   }
 }
 
+// ---------------------------------------------------------------------------
+// Symmetric FP64 k = 1 product of the square operator (the refinement
+// residual).  Workgroup (b, seg) owns rows [S64_RB b, S64_RB (b + 1)) (4 waves
+// x 4 groups of 16 rows) and the 16-column tiles of segment seg at or right of
+// tile S64_DT b, so each exponential is evaluated once for the pair (i, j),
+// (j, i).  Gram on v_mfma_f64_16x16x4f64, exponential by exp2_t256.  Lane
+// (m, kq) holds P[j = J0 + kq + 4 r][i = I0 + 16 it + m]: row sums (weights
+// x_j) accumulate in-lane; column sums (weights x_i) are summed over the row
+// groups in-lane, over the 16 lanes m by DPP within the row, over the 4 waves
+// in LDS (fixed order), and stored as one FP64 partial per (row block, column)
+// right of the diagonal block: part_c[coff(b) + j - S64_RB (b + 1)],
+// coff(b) = b n - (S64_RB / 2) b (b + 1).  Row partials: part_r[seg][i].
+// ---------------------------------------------------------------------------
+constexpr int S64_RB = 256;             // rows per workgroup
+constexpr int S64_SEG = 128;            // 16-column tiles per segment
+constexpr int S64_DT = S64_RB / 16;     // tiles of the diagonal block
+constexpr int S64_CH = 8;               // tiles per column-sum chunk (one barrier pair)
+
+__host__ __device__ inline int64_t s64_coff(int64_t b, int64_t n) { return b * n - (S64_RB / 2) * b * (b + 1); }
+
+template <int CTRL>
+__device__ inline double dpp_d(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xF, 0xF, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Sum over the 16 lanes of a DPP row (every lane receives it): quad_perm
+// [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror.
+__device__ inline double row_sum16(double v) {
+  v += dpp_d<0xB1>(v);
+  v += dpp_d<0x4E>(v);
+  v += dpp_d<0x141>(v);
+  v += dpp_d<0x140>(v);
+  return v;
+}
+
+template <int STEPS, bool CLAMP, class XT>
+__global__ __launch_bounds__(256) void mv_sym64_kernel(const double* __restrict__ E, int d, int64_t n,
+                                                        double rscale, double clampv,
+                                                        const XT* __restrict__ X, int64_t ldx,
+                                                        const int2* __restrict__ map,
+                                                        double* __restrict__ part_r,
+                                                        double* __restrict__ part_c) {
+  __shared__ double tab[256];
+  __shared__ double colbuf[4][S64_CH * 16];
+  tab[threadIdx.x] = exp2((double)threadIdx.x * (1.0 / 256.0));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int m = lane & 15, kq = lane >> 4;
+  const int2 bs = map[blockIdx.x];
+  const int64_t b = bs.x;
+  const int64_t I0 = b * S64_RB + w * 64;
+  const bool active = I0 < n;
+  const int64_t wrem = (n - b * S64_RB + 63) / 64;
+  const int nact = wrem < 4 ? (int)wrem : 4;
+  double bo[4][STEPS], xr[4];
+#pragma unroll
+  for (int it = 0; it < 4; ++it) {
+    const int64_t i = I0 + 16 * it + m;
+#pragma unroll
+    for (int s = 0; s < STEPS; ++s) {
+      const int k = kq + 4 * s;
+      bo[it][s] = (i < n && k < d) ? E[i * d + k] * rscale : 0.0;
+    }
+    xr[it] = i < n ? (double)X[i * ldx] : 0.0;
+  }
+  const int64_t ntiles = (n + 15) / 16;
+  const int64_t tdiag = S64_DT * (b + 1);  // first tile right of the diagonal block
+  const int64_t t0 = (S64_DT * b > (int64_t)bs.y * S64_SEG) ? S64_DT * b : (int64_t)bs.y * S64_SEG;
+  const int64_t t1 = ((int64_t)bs.y * S64_SEG + S64_SEG < ntiles) ? (int64_t)bs.y * S64_SEG + S64_SEG : ntiles;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  __syncthreads();  // table
+  for (int64_t tc = t0; tc < t1; tc += S64_CH) {
+    const int nt = (int)((t1 - tc) < S64_CH ? (t1 - tc) : S64_CH);
+    if (active) {
+      for (int tt = 0; tt < nt; ++tt) {
+        const int64_t J0 = (tc + tt) * 16;
+        double a[STEPS];
+#pragma unroll
+        for (int s = 0; s < STEPS; ++s) {
+          const int64_t j = J0 + m;
+          const int k = kq + 4 * s;
+          a[s] = (j < n && k < d) ? E[j * d + k] : 0.0;
+        }
+        double xv[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t j = J0 + kq + 4 * r;
+          xv[r] = j < n ? (double)X[j * ldx] : 0.0;
+        }
+        double q[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int it = 0; it < 4; ++it) {
+          doublex4 D = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int s = 0; s < STEPS; ++s) D = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], bo[it][s], D, 0, 0, 0);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            double tp = D[r];
+            if constexpr (CLAMP) tp = fmin(tp, clampv);
+            const double ev = exp2_t256(tp, tab);
+            acc[it] = __builtin_fma(ev, xv[r], acc[it]);
+            q[r] = __builtin_fma(ev, xr[it], q[r]);
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) q[r] = row_sum16(q[r]);
+        if (m == 0) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) colbuf[w][tt * 16 + kq + 4 * r] = q[r];
+        }
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < nt * 16) {
+      const int64_t t = tc + (threadIdx.x >> 4);
+      const int64_t j = t * 16 + (threadIdx.x & 15);
+      if (t >= tdiag && j < n) {
+        double v = 0.0;
+        for (int ww = 0; ww < nact; ++ww) v += colbuf[ww][threadIdx.x];
+        part_c[s64_coff(b, n) + j - S64_RB * (b + 1)] = v;
+      }
+    }
+    __syncthreads();
+  }
+  if (!active) return;
+#pragma unroll
+  for (int it = 0; it < 4; ++it) {
+    double v = acc[it];
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    const int64_t i = I0 + 16 * it + m;
+    if (kq == 0 && i < n) part_r[(int64_t)bs.y * n + i] = v;
+  }
+}
+
+// s_i = lam xr_i + sum_seg part_r[seg][i] + sum_{b < i / S64_RB} part_c[coff(b) + i - S64_RB (b + 1)];
+// Y_i = B_i - s_i (residual) or s_i
+template <class XT>
+__global__ void mv_sym64_reduce_kernel(const double* __restrict__ part_r, const double* __restrict__ part_c,
+                                       int64_t n, double lam, const XT* __restrict__ Xr, int64_t ldxr,
+                                       const double* __restrict__ B, double* __restrict__ Y, int64_t ldy) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t ntiles = (n + 15) / 16;
+  const int64_t b = i / S64_RB;
+  const int64_t nseg = (ntiles + S64_SEG - 1) / S64_SEG;
+  double a8[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double tail = 0.0;
+  int64_t sg = (S64_DT * b) / S64_SEG;  // first segment of row block b
+  for (; sg + 8 <= nseg; sg += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a8[u] += part_r[(sg + u) * n + i];
+  }
+  for (; sg < nseg; ++sg) tail += part_r[sg * n + i];
+  int64_t bb = 0;
+  for (; bb + 8 <= b; bb += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int64_t q = bb + u;
+      a8[u] += part_c[s64_coff(q, n) + i - S64_RB * (q + 1)];
+    }
+  }
+  for (; bb < b; ++bb) tail += part_c[s64_coff(bb, n) + i - S64_RB * (bb + 1)];
+  double s = (((a8[0] + a8[1]) + (a8[2] + a8[3])) + ((a8[4] + a8[5]) + (a8[6] + a8[7]))) + tail;
+  if (Xr) s += lam * (double)Xr[i * ldxr];
+  Y[i * ldy] = B ? B[i * ldy] - s : s;
+}
+
 // Y[i][c] = lam * Xr[i][c] + sum_js part[js][i][c]     (Xr may be null)
 // MODE_RES: Y = B - lam*Xr - sum   (FP64 residual)
 template <class PT, class XT, class YT, bool RES>
@@ -1693,10 +1863,45 @@ int fused_product_f64(Operator& op, const Side& rows, const Side& cols, const XT
   while (ceil_div(n_rows, 256) * jsplit < 1024 && jsplit < 64) jsplit *= 2;
   while (jsplit > 1 && ceil_div(n_cols, 64) / jsplit < 2) jsplit /= 2;
   const int64_t ldp = k;
-  LAKER_TRY(op.ensure_part((size_t)jsplit * n_rows * k * sizeof(double)));
-  double* part = op.part.as<double>();
   // t' = 256 s log2(e) g must stay inside int range for the table exp2
   const double tmax = op.max_sq_norm * std::fabs(op.scale) * kLog2eD * 256.0;
+  if (k == 1 && op.sym_enabled && op.sym64_nwg > 0 && rows.e64 == cols.e64 && n_rows == n_cols &&
+      op.d <= 16 && (clamp || tmax < 1e9)) {
+    // square operator, one right-hand side: each exponential once per pair
+    const int steps = (op.d + 3) / 4;
+    const double rscale = op.scale * kLog2eD * 256.0;
+    const double clampv = op.clamp_limit * kLog2eD * 256.0;
+    const int64_t n = n_rows;
+    const int64_t nseg = ceil_div(ceil_div(n, 16), S64_SEG);
+    const int64_t ncb = ceil_div(n, S64_RB) - 1;
+    LAKER_TRY(op.ensure_part((size_t)nseg * n * sizeof(double)));
+    LAKER_TRY(op.part_c64.ensure((size_t)std::max<int64_t>(1, s64_coff(ncb, n)) * sizeof(double)));
+    double* part_r = op.part.as<double>();
+    double* part_c = op.part_c64.as<double>();
+    const int2* map = op.sym64_map.as<int2>();
+    dim3 grid((unsigned)op.sym64_nwg), block(256);
+#define LAKER_S64(ST, CL)                                                                              \
+  hipLaunchKernelGGL((mv_sym64_kernel<ST, CL, XT>), grid, block, 0, st, rows.e64, op.d, n, rscale, clampv, \
+                     X, ldx, map, part_r, part_c)
+#define LAKER_S64_ST(CL)                     \
+  do {                                       \
+    if (steps == 1) LAKER_S64(1, CL);        \
+    else if (steps == 2) LAKER_S64(2, CL);   \
+    else if (steps == 3) LAKER_S64(3, CL);   \
+    else LAKER_S64(4, CL);                   \
+  } while (0)
+    if (clamp) LAKER_S64_ST(true);
+    else LAKER_S64_ST(false);
+#undef LAKER_S64_ST
+#undef LAKER_S64
+    LAKER_KCHECK("mv_sym64_kernel");
+    hipLaunchKernelGGL((mv_sym64_reduce_kernel<XT>), dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st,
+                       (const double*)part_r, (const double*)part_c, n, lam, Xr, ldxr, B, Y, ldy);
+    LAKER_KCHECK("mv_sym64_reduce_kernel");
+    return LAKER_OK;
+  }
+  LAKER_TRY(op.ensure_part((size_t)jsplit * n_rows * k * sizeof(double)));
+  double* part = op.part.as<double>();
   if (op.d <= 16 && (clamp || tmax < 1e9)) {
     const int steps = (op.d + 3) / 4;
     const double rscale = op.scale * kLog2eD * 256.0;
@@ -1860,6 +2065,21 @@ int laker_operator_create(const void* E, int64_t n, int32_t d, int32_t dtype, do
   op->sym_nwg = (int)smap.size();
   if (e == hipSuccess)
     e = hipMemcpyAsync(op->sym_map.ptr, smap.data(), smap.size() * sizeof(int2), hipMemcpyHostToDevice, stream);
+  // workgroup map of the symmetric FP64 k = 1 product (256-row blocks, 2048-column segments)
+  std::vector<int2> smap64;
+  {
+    const int64_t ntiles = ceil_div(n, 16), nb = ceil_div(n, S64_RB), nseg = ceil_div(ntiles, S64_SEG);
+    for (int64_t sg = 0; sg < nseg; ++sg)
+      for (int64_t b = 0; b < nb && S64_DT * b < S64_SEG * (sg + 1); ++b) smap64.push_back(int2{(int)b, (int)sg});
+  }
+  if (e == hipSuccess && (rc = op->sym64_map.ensure(smap64.size() * sizeof(int2))) != LAKER_OK) {
+    delete op;
+    return rc;
+  }
+  op->sym64_nwg = (int)smap64.size();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(op->sym64_map.ptr, smap64.data(), smap64.size() * sizeof(int2), hipMemcpyHostToDevice,
+                       stream);
   uint64_t bits = 0;
   if (e == hipSuccess) e = hipMemcpyAsync(&bits, mx.ptr, 8, hipMemcpyDeviceToHost, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
