@@ -567,6 +567,32 @@ __global__ __launch_bounds__(256) void gemv64_kernel(const double* __restrict__ 
   if (lane == 0) y[r] = s;
 }
 
+// y[r] = sum_c S[r][c] w1[c] with w1 = the sum of the 8 group partials of
+// Q^T x, formed per block in LDS in the order of reduce_parts_groups_kernel
+// ((g0 + g4) + (g1 + g5)) + ((g2 + g6) + (g3 + g7)), so the result matches
+// the separate reduction bit for bit.  Two rows per wave.
+__global__ __launch_bounds__(256) void gemv64_sum8_kernel(const double* __restrict__ S,
+                                                          const double* __restrict__ part8,
+                                                          double* __restrict__ y, int nr) {
+  extern __shared__ double w1s[];
+  for (int c = threadIdx.x; c < nr; c += 256) {
+    const double* q = part8 + c;
+    w1s[c] = ((q[0] + q[4 * nr]) + (q[nr] + q[5 * nr])) + ((q[2 * nr] + q[6 * nr]) + (q[3 * nr] + q[7 * nr]));
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int rr = 0; rr < 2; ++rr) {
+    const int r = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 2 + rr;
+    if (r < nr) {  // uniform per wave
+      double v = 0.0;
+      for (int c = lane; c < nr; c += 64) v = fma(S[(int64_t)r * nr + c], w1s[c], v);
+      v = wave_sum(v);
+      if (lane == 0) y[r] = v;
+    }
+  }
+}
+
 // z[i] = f * x[i] + sum_c Q[i][c] w[c]   (one wave per QW_ROWS rows)
 constexpr int QW_ROWS = 2;
 template <class T>
@@ -635,12 +661,10 @@ static int apply_t(Precond& pc, const T* X, int k, int64_t ldx, T* Z, int64_t ld
       hipLaunchKernelGGL(reduce_parts_groups_kernel, dim3(cb, RG), dim3(256), 0, st, (const double*)part,
                          nb_eff, nr, part2);
       LAKER_KCHECK("reduce_parts_groups_kernel");
-      hipLaunchKernelGGL(reduce_parts_groups_kernel, dim3(cb, 1), dim3(256), 0, st, (const double*)part2, RG,
-                         nr, W1);
-      LAKER_KCHECK("reduce_parts_groups_kernel");
-      hipLaunchKernelGGL(gemv64_kernel, dim3((unsigned)ceil_div(nr, 4)), dim3(256), 0, st, pc.S.as<double>(),
-                         W1, W2, nr);
-      LAKER_KCHECK("gemv64_kernel");
+      static_assert(RG == 8, "gemv64_sum8_kernel sums eight group partials");
+      hipLaunchKernelGGL(gemv64_sum8_kernel, dim3((unsigned)ceil_div(nr, 8)), dim3(256), nr * sizeof(double), st,
+                         pc.S.as<double>(), (const double*)part2, W2, nr);
+      LAKER_KCHECK("gemv64_sum8_kernel");
       hipLaunchKernelGGL((qw4_kernel<2>), dim3((unsigned)ceil_div(n, 8)), dim3(256), nr * sizeof(double), st,
                          Q, (const double*)W2, X, Z, n, nr, pc.inv_sqrt_a);
       LAKER_KCHECK("qw4_kernel");

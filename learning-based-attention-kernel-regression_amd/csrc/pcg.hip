@@ -150,6 +150,29 @@ __global__ void pcg_update_xr_kernel(TX* __restrict__ x, T* __restrict__ r, cons
   r[idx] = (T)((double)r[idx] - a * (double)ap[idx]);
 }
 
+// k = 1: x += alpha p ; r -= alpha Ap, and part[blk] = sum over the block's
+// 256 rows of r_new^2 (the partials of ||r||^2 for pcg_resid_kernel)
+template <class T>
+__global__ __launch_bounds__(256) void pcg_update_xr_rr_kernel(T* __restrict__ x, T* __restrict__ r,
+                                                               const T* __restrict__ p, const T* __restrict__ ap,
+                                                               const double* __restrict__ alpha, int64_t n,
+                                                               double* __restrict__ part,
+                                                               const PcgState* __restrict__ st) {
+  if (st->done) return;  // written by an earlier kernel: uniform over the grid
+  __shared__ double red[16];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double rr = 0.0;
+  if (i < n) {
+    const double a = alpha[0];
+    x[i] = (T)((double)x[i] + a * (double)p[i]);
+    const T rn = (T)((double)r[i] - a * (double)ap[i]);
+    r[i] = rn;
+    rr = (double)rn * (double)rn;
+  }
+  const double s = block_sum(rr, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
 // residual norm, convergence and max-iteration bookkeeping.  One block.
 __global__ void pcg_resid_kernel(const double* __restrict__ part, int nblk, int k,
                                  PcgState* __restrict__ st) {
@@ -296,7 +319,9 @@ static int pcg_buffers(Operator& op, int k, PcgBuffers<T>& b) {
   int64_t rpb;
   const int nblk = pcg_nblk(op.n, &rpb);
   const size_t vec_bytes = round_up(nk * sizeof(T), 256);
-  const size_t part_bytes = round_up((size_t)3 * nblk * k * 8, 256);
+  // coldots partials (3 x nblk x k) or, for k = 1, the per-256-row partials of
+  // the fused update kernels
+  const size_t part_bytes = round_up(std::max<size_t>((size_t)3 * nblk * k, (size_t)ceil_div(op.n, 256)) * 8, 256);
   const size_t col_bytes = round_up((size_t)k * 8, 256);
   const size_t need = 5 * vec_bytes + part_bytes + 3 * col_bytes + 256;
   LAKER_TRY(op.pcg_ws.ensure(need));
@@ -426,14 +451,26 @@ static int pcg_run(Operator& op, Precond* pc, const T* B, int k, T* X, const T* 
     LAKER_TRY(coldots<T>(b, 3, b.p, b.ap, b.p, b.p, b.ap, b.ap, n, k, true, st));
     hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.alpha, b.st);
     LAKER_KCHECK("pcg_alpha_kernel");
-    hipLaunchKernelGGL((pcg_update_xr_kernel<T, T>), eg, eb, 0, st, b.x, b.r, b.p, b.ap, b.alpha, n, k,
-                       b.st);
-    LAKER_KCHECK("pcg_update_xr_kernel");
+    if (k == 1) {
+      // ||r||^2 partials come out of the update itself (restarted runs redo
+      // them after the residual replacement in tail)
+      hipLaunchKernelGGL((pcg_update_xr_rr_kernel<T>), eg, eb, 0, st, b.x, b.r, (const T*)b.p, (const T*)b.ap,
+                         (const double*)b.alpha, n, b.part, (const PcgState*)b.st);
+      LAKER_KCHECK("pcg_update_xr_rr_kernel");
+    } else {
+      hipLaunchKernelGGL((pcg_update_xr_kernel<T, T>), eg, eb, 0, st, b.x, b.r, b.p, b.ap, b.alpha, n, k,
+                         b.st);
+      LAKER_KCHECK("pcg_update_xr_kernel");
+    }
     return LAKER_OK;
   };
   auto tail = [&]() -> int {
-    LAKER_TRY(coldots<T>(b, 1, b.r, b.r, nullptr, nullptr, nullptr, nullptr, n, k, true, st));
-    hipLaunchKernelGGL(pcg_resid_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.st);
+    if (k == 1 && restart_freq <= 0) {
+      hipLaunchKernelGGL(pcg_resid_kernel, dim3(1), dim3(256), 0, st, b.part, (int)eg.x, 1, b.st);
+    } else {
+      LAKER_TRY(coldots<T>(b, 1, b.r, b.r, nullptr, nullptr, nullptr, nullptr, n, k, true, st));
+      hipLaunchKernelGGL(pcg_resid_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.st);
+    }
     LAKER_KCHECK("pcg_resid_kernel");
     if (pc) LAKER_TRY(precond_apply(*pc, b.r, k, k, b.z, k, st));
     else hipLaunchKernelGGL((copy_kernel<T>), eg, eb, 0, st, (const T*)b.r, b.z, nk);
@@ -523,6 +560,29 @@ __global__ void scale_to_f32_kernel(const double* __restrict__ a, float* __restr
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) out[i] = (float)(a[i] * f);
 }
+// k = 1: x (FP64) += f alpha p ; r -= alpha Ap, and part[blk] = sum over the
+// block's 256 rows of r_new^2
+__global__ __launch_bounds__(256) void pcg_update_x64_rr_kernel(double* __restrict__ x, float* __restrict__ r,
+                                                                const float* __restrict__ p,
+                                                                const float* __restrict__ ap,
+                                                                const double* __restrict__ alpha, int64_t n,
+                                                                double* __restrict__ part,
+                                                                const PcgState* __restrict__ st) {
+  if (st->done) return;  // written by an earlier kernel: uniform over the grid
+  __shared__ double red[16];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double rr = 0.0;
+  if (i < n) {
+    const double a = alpha[0];
+    x[i] += st->xscale * a * (double)p[i];
+    const float rn = (float)((double)r[i] - a * (double)ap[i]);
+    r[i] = rn;
+    rr = (double)rn * (double)rn;
+  }
+  const double s = block_sum(rr, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
 // x (FP64) += f * alpha p ; r -= alpha Ap  (FP32 recursion on a scaled residual)
 __global__ void pcg_update_x64_kernel(double* __restrict__ x, float* __restrict__ r,
                                       const float* __restrict__ p, const float* __restrict__ ap,
@@ -634,11 +694,18 @@ int laker_solve_mixed(laker_operator_t oh, laker_precond_t ph, const double* B, 
     LAKER_TRY(coldots<float>(b, 3, b.p, b.ap, b.p, b.p, b.ap, b.ap, n, k, true, st));
     hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.alpha, b.st);
     LAKER_KCHECK("pcg_alpha_kernel");
-    hipLaunchKernelGGL(pcg_update_x64_kernel, eg, eb, 0, st, X, b.r, (const float*)b.p, (const float*)b.ap,
-                       (const double*)b.alpha, n, k, (const PcgState*)b.st);
-    LAKER_KCHECK("pcg_update_x64_kernel");
-    LAKER_TRY(coldots<float>(b, 1, b.r, b.r, nullptr, nullptr, nullptr, nullptr, n, k, true, st));
-    hipLaunchKernelGGL(pcg_resid_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.st);
+    if (k == 1) {
+      hipLaunchKernelGGL(pcg_update_x64_rr_kernel, eg, eb, 0, st, X, b.r, (const float*)b.p,
+                         (const float*)b.ap, (const double*)b.alpha, n, b.part, (const PcgState*)b.st);
+      LAKER_KCHECK("pcg_update_x64_rr_kernel");
+      hipLaunchKernelGGL(pcg_resid_kernel, dim3(1), dim3(256), 0, st, b.part, (int)eg.x, 1, b.st);
+    } else {
+      hipLaunchKernelGGL(pcg_update_x64_kernel, eg, eb, 0, st, X, b.r, (const float*)b.p, (const float*)b.ap,
+                         (const double*)b.alpha, n, k, (const PcgState*)b.st);
+      LAKER_KCHECK("pcg_update_x64_kernel");
+      LAKER_TRY(coldots<float>(b, 1, b.r, b.r, nullptr, nullptr, nullptr, nullptr, n, k, true, st));
+      hipLaunchKernelGGL(pcg_resid_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.st);
+    }
     LAKER_KCHECK("pcg_resid_kernel");
     if (pc) LAKER_TRY(precond_apply(*pc, b.r, k, k, b.z, k, st));
     else hipLaunchKernelGGL((copy_kernel<float>), eg, eb, 0, st, (const float*)b.r, b.z, nk);

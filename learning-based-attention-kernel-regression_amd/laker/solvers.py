@@ -248,8 +248,12 @@ class PreconditionedConjugateGradient:
         self.converged = bool(res.converged)
         if self.verbose:
             rel = self.residual_norm / bnorm if bnorm else 0.0
-            logger.info("mixed PCG: %d outer / %d inner iterations, fp64 rel_res=%.3e",
-                        self.outer_iterations, self.iterations, rel)
+            if self.converged:
+                logger.info("mixed PCG: %d outer / %d inner iterations, fp64 rel_res=%.3e",
+                            self.outer_iterations, self.iterations, rel)
+            else:
+                # as the reference's non-convergence warning (laker/solvers.py:288-294)
+                logger.warning("PCG did not converge in %d iterations, rel_res=%.3e", self.iterations, rel)
         x = x.squeeze(1) if rhs.dim() == 1 else x
         self.solution_fp64 = x
         return x.to(device=out_dev, dtype=rhs.dtype if rhs.dtype.is_floating_point else torch.float32)
