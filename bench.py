@@ -77,6 +77,11 @@ def parse():
     ap.add_argument("--serving-requests", type=int, default=24,
                     help="requests in the bounded serving sample (0 disables it)")
     ap.add_argument("--serving-slots", type=int, default=4)
+    ap.add_argument("--serving-rate", type=float, default=0.0,
+                    help="Poisson arrival rate (requests/s, seed 0) of the serving sample, honoured against "
+                         "the wall clock; 0 makes every request available at t=0 (saturating)")
+    ap.add_argument("--serving-seq", type=int, default=0,
+                    help="requests of the sequential fp32 comparison (0: the same count as the sample)")
     ap.add_argument("--quick", action="store_true")
     return ap.parse_args()
 
@@ -321,26 +326,32 @@ def main():
 
     # Bounded sample of the serving stream: requests r = 0..R-1 (n log-uniform in
     # [1000, 65536], d in {10, 16}, 1..32 RHS, all seeded by r), all available at
-    # t=0 (saturating), continuous batching over the slots; against the same
-    # requests solved one after another in plain fp32.
+    # t=0 (saturating) or arriving as a Poisson stream (--serving-rate),
+    # continuous batching over the slots; against the same requests (or the
+    # first --serving-seq of them) solved one after another in plain fp32.
     serving = None
     if args.serving_requests > 0:
-        from laker.serving import SolveServer, make_request
+        from laker.serving import SolveServer, make_request, poisson_arrivals
 
-        reqs = [make_request(r) for r in range(args.serving_requests)]
+        realtime = args.serving_rate > 0
+        arr = poisson_arrivals(args.serving_requests, args.serving_rate, seed=0) if realtime else \
+            [0.0] * args.serving_requests
+        reqs = [make_request(r, arrival=arr[r]) for r in range(args.serving_requests)]
         srv = SolveServer(slots=args.serving_slots, precision="mixed", tol=args.tol, inner_tol=args.inner_tol)
         srv.run(reqs[:2], realtime=False)  # warm-up (embeddings, code objects)
-        res_s = srv.run(reqs, realtime=False)
+        res_s = srv.run(reqs, realtime=realtime)
         seq = SolveServer(slots=1, precision="fp32", tol=args.tol)
         seq.run(reqs[:2], realtime=False)
-        res_q = seq.run(reqs, realtime=False)
+        res_q = seq.run(reqs[:args.serving_seq or len(reqs)], realtime=False)
         sm, sq = SolveServer.summary(res_s, srv.elapsed_s), SolveServer.summary(res_q, seq.elapsed_s)
         worst = max((r.relres for r in res_s if r.error is None), default=float("nan"))
-        serving = {"requests": len(reqs), "slots": args.serving_slots, "arrivals": "all at t=0 (saturating)",
+        serving = {"requests": len(reqs), "slots": args.serving_slots,
+                   "arrivals": (f"Poisson, {args.serving_rate}/s, seed 0, wall clock" if realtime
+                                else "all at t=0 (saturating)"),
                    "n_range": [min(r.n for r in reqs), max(r.n for r in reqs)],
                    "rhs_total": sum(r.k for r in reqs), "max_inflight": srv.max_inflight,
                    "mixed_continuous": sm, "max_fp64_relres": worst,
-                   "fp32_sequential": sq}
+                   "fp32_sequential": sq, "fp32_sequential_requests": len(res_q)}
         print(f"[bench] serving: {sm.get('solves_per_sec', 0):.2f} solves/s p50 {sm.get('p50_ms', 0):.0f} ms "
               f"p99 {sm.get('p99_ms', 0):.0f} ms; sequential fp32 {sq.get('solves_per_sec', 0):.2f} solves/s",
               file=sys.stderr, flush=True)
