@@ -89,8 +89,8 @@ def parse():
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU: multi-GPU runs are not part of this build "
-              "(the sharding design is in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the solve shards by output rows across GPUs by design, "
+              "but only the one-GPU path is implemented; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.quick:
         args.n, args.oracle_n = 4000, 2000

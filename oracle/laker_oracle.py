@@ -20,7 +20,7 @@ Pinning (what checks each function independently of this restatement):
 * ``pcg_solve``: exact-preconditioner one-step convergence, diagonal SPD
   systems against closed forms, FP64 Cholesky agreement, breakdown on an
   indefinite diagonal, Frobenius stopping rule on synthetic columns.
-* ``adaptive_shrinkage_rho``: closed-form values (C0 -> 0.5).
+* ``adaptive_shrinkage_rho``: closed-form values (n = 2000, N_r = 200, gamma = 0.1 -> 0.5).
 * ``cccp_build`` / ``cccp_apply``: invariants (a > 0, tr Sigma = n, M SPD,
   P linear and symmetric, P == Sigma^{-1/2} by dense eigh at small n),
   an FP64 NumPy restatement with a different factorisation route

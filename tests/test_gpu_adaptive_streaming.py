@@ -101,7 +101,7 @@ def test_regressor_adaptive_strategy(cuda):
     m = laker.LAKERRegressor(verbose=False, preconditioner="adaptive").fit(x, y)
     assert m.preconditioner.inner_name in ("cccp", "cccp_aggressive")
     ref = laker.LAKERRegressor(verbose=False).fit(x, y)
-    # C0's conditioning selects CCCP on the same default probes: the same factors and solve
+    # the n = 2000 problem's conditioning selects CCCP on the same default probes: the same factors and solve
     assert m.preconditioner.inner.isotropic_coef == pytest.approx(ref.preconditioner.isotropic_coef, rel=1e-12)
     assert torch.equal(m.alpha.cpu(), ref.alpha.cpu())
 
