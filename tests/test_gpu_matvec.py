@@ -133,10 +133,12 @@ def test_symmetric_product_matches_general(cuda, n, d, precision):
     assert np.array_equal(op.matvec(xt).double().cpu().numpy(), y_sym)
 
 
-@pytest.mark.parametrize("n,amp", [(3000, 0.6), (1500, 9.0)])
+@pytest.mark.parametrize("n,amp", [(3000, 0.6), (1500, 9.0), (1, 0.6), (17, 0.6), (257, 0.6), (1025, 0.6),
+                                   (2049, 9.0)])
 def test_residual_fp64_single_rhs(cuda, n, amp):
-    """k=1 FP64 residual (FP64 MFMA Gram + table exp2) against NumPy FP64,
-    with and without the clamp."""
+    """k=1 FP64 residual (symmetric FP64 MFMA Gram + table exp2, each exp once
+    per pair) against NumPy FP64, with and without the clamp, on sizes that
+    leave partial 16-row tiles and partial 256-row blocks."""
     e, _ = make(n, 10, 1, seed=n, amp=amp)
     rng = np.random.default_rng(7)
     x = rng.standard_normal(n) * 50
