@@ -5,7 +5,7 @@ system (one GPU).
 A step is one complete solve of (lambda I + exp(E E^T)) alpha = y with E and
 y already resident in HBM: operator packing, CCCP build (probe matvec, QR,
 FP64 CCCP loop, inverse square root) and the PCG solve.
-  mixed: BF16 probe matvec, BF16 hi/lo-split Gram inner PCG, FP64 iterative
+  mixed: BF16 probe matvec, split-Gram inner PCG (FP16 2-way split for the k=1 products), FP64 iterative
          refinement until the FP64 true residual ||y - A alpha|| / ||y|| <= 1e-6
   fp32:  FP32 probe matvec and FP32-MFMA-Gram PCG until the recurrence
          residual <= 1e-6 (the reference's stopping rule)
@@ -225,8 +225,8 @@ def main():
     # costs one exp and two FMAs (row sum with x_j, column sum with x_i)
     valu_floor_ms = (n * n / 2) / 64 * (EXP_CYCLES + 2 * FMA_CYCLES) / N_SIMD / CLOCK_HZ * 1e3
     roofline = {
-        "kernel": "mv_sym1_kernel<LAKER_PREC_MIXED> + mv_sym_reduce_kernel (laker_matvec, k=1, "
-                  "BF16-split Gram on MFMA, each exp once per (i,j),(j,i) pair)",
+        "kernel": "mv_sym1_kernel<GRAM_F16S> + mv_sym_reduce_kernel (laker_matvec, k=1, "
+                  "FP16 2-way split Gram on MFMA, each exp once per (i,j),(j,i) pair)",
         "bound": "mfma", "unit": "TFLOP/s",
         "achieved": flops_mv / t_mv / 1e12, "peak": MFMA_BF16_PEAK_TFLOPS,
         "frac": flops_mv / t_mv / 1e12 / MFMA_BF16_PEAK_TFLOPS,
@@ -367,7 +367,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16-gram/fp32-pcg/fp64-refine",
+        "dtype": "fp16-split-gram/fp32-pcg/fp64-refine",
         "data": "synthetic (path-loss field, PositionEmbedding default init, seed 0)",
         "config": {"workload": f"large matrix-free solve n={n} d={d} k=1 lambda={args.lambda_reg} "
                                f"CCCP N_r={nr} tol={args.tol}",

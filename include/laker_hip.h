@@ -37,8 +37,10 @@ extern "C" {
 
 /* ---- matvec arithmetic */
 #define LAKER_PREC_FP32 0  /* Gram on FP32 MFMA, FP32 exp and products */
-#define LAKER_PREC_MIXED 1 /* Gram on BF16 MFMA from a 3-way bf16 split of E (6 products); for k > 4 the
-                              exp tile times X also on BF16 MFMA from 3-way splits (6 products, ~2^-24) */
+#define LAKER_PREC_MIXED 1 /* Gram on BF16 MFMA from a 3-way bf16 split of E (6 products, ~2^-24); the
+                              square operator's k = 1 product instead uses a 2-way FP16 split (3 products,
+                              ~2^-22) while max |e| * max(1, scale log2 e) < 2^14; for k > 4 the exp tile
+                              times X also on BF16 MFMA from 3-way splits (6 products, ~2^-24) */
 #define LAKER_PREC_FP64 2  /* Gram, exp and accumulation in FP64 (refinement residual) */
 #define LAKER_PREC_BF16 3  /* split Gram; for k > 4 one BF16 product of the exp tile and X (~2^-8) */
 #define LAKER_PREC_HILO 4  /* split Gram; for k > 4 hi/lo BF16 products (3 products, ~2^-16):
@@ -61,8 +63,9 @@ const char* laker_version(void);
  * Exact attention-kernel operator  A = lambda I + exp(scale * E E^T)
  * Replaces AttentionKernelOperator.__init__ (laker/kernels.py:154-185).
  * E: n x d row-major (ld = d), dtype LAKER_F32 or LAKER_F64, device memory.
- * The handle copies and packs E (FP32 MFMA operands, BF16 hi/lo split
- * operands, FP64 copy) and evaluates the skip-clamp predicate on device.
+ * The handle copies and packs E (FP32 MFMA operands, BF16 3-way split and
+ * FP16 2-way split operands, FP64 copy) and evaluates the skip-clamp
+ * predicate on device.
  */
 int laker_operator_create(const void* E, int64_t n, int32_t d, int32_t dtype, double lambda_reg,
                           double scale, int32_t clamp_policy, hipStream_t stream,
@@ -222,7 +225,7 @@ int laker_pcg_solve(laker_operator_t op, laker_precond_t pc, const void* B, int3
 
 /* Mixed-precision solve (iterative refinement): each outer step runs PCG
  * from a fresh start on the FP64 residual B - A X scaled to unit norm, with
- * operator products on BF16-split MFMA Gram tiles (LAKER_PREC_MIXED, or FP32
+ * operator products on split MFMA Gram tiles (LAKER_PREC_MIXED, or FP32
  * MFMA with LAKER_PREC_FP32), FP32 vectors and FP64 scalars, and adds its
  * correction straight into the FP64 X.  A step stops when its recursive
  * residual has dropped by inner_tol (or by what the outer target still

@@ -19,6 +19,7 @@ struct Side {
 struct SideStore {
   int64_t n = 0, npad = 0;
   DevBuf col32, row32, col16, row16, e64, user;
+  DevBuf col16h, row16h;  // FP16 2-way split operands (square operator only, ka16h halves)
 };
 
 struct Operator {
@@ -32,6 +33,8 @@ struct Operator {
   double max_sq_norm = 0.0;
   bool skip_clamp = true;
   int ka32 = 0, ka16 = 0;
+  int ka16h = 0;         // round_up(3d, 16): FP16 2-way split operand width
+  bool f16s_ok = false;  // the scaled embedding fits FP16: k = 1 square products use the FP16 split
   SideStore self;
   SideStore query;  // packed query set of the last cross product
   std::mutex query_mu;
@@ -71,6 +74,7 @@ struct Operator {
 };
 
 // Pack an embedding set (row side pre-scaled by scale*log2(e)).
+int pack_f16_split(SideStore& s, const void* E, int dtype, int64_t n, int d, int ka, double scale, hipStream_t st);
 int pack_store(SideStore& s, const void* E, int dtype, int64_t n, int d, int ka32, int ka16,
                double scale, bool need_col, bool need_row, hipStream_t st);
 

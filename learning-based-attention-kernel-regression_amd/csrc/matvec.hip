@@ -34,6 +34,11 @@ namespace laker {
 //          col side [hi|hi|mid|hi|lo|mid], row side [hi'|mid'|hi'|lo'|hi'|mid'],
 //          so the K=6d contraction is the six leading products of the split
 //          (about 2^-24 relative, FP32 accumulation in the MFMA).
+//   FP16 split (square operator's k = 1 product): ka = round_up(3d,16) halves
+//          per row from e = hi + lo: col side [hi|hi|lo], row side
+//          [hi'|lo'|hi'], the three leading products (22 significant bits
+//          per operand, FP32 accumulation); used only when max |e| and
+//          max |c e| stay far below the FP16 range (2^14).
 // Rows are padded with zeros to a multiple of 64.
 // ---------------------------------------------------------------------------
 
@@ -82,6 +87,32 @@ __global__ void pack_bf16x4_kernel(const TE* __restrict__ E, int64_t n, int d, i
   }
   if (colside) colside[idx] = cv;
   if (rowside) rowside[idx] = rv;
+}
+
+template <class TE>
+__global__ void pack_f16x2_kernel(const TE* __restrict__ E, int64_t n, int d, int64_t npad, int ka,
+                                  float cscale, uint16_t* __restrict__ colside, uint16_t* __restrict__ rowside) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= npad * ka) return;
+  int64_t i = idx / ka;
+  int p = (int)(idx % ka);
+  uint16_t cv = 0, rv = 0;
+  if (i < n && p < 3 * d) {
+    const int seg = p / d, dim = p % d;
+    const float e = (float)E[i * d + dim];
+    const _Float16 hi = (_Float16)e;
+    const _Float16 lo = (_Float16)(e - (float)hi);
+    const float es = e * cscale;
+    const _Float16 his = (_Float16)es;
+    const _Float16 los = (_Float16)(es - (float)his);
+    // three products: hi.hi' + hi.lo' + lo.hi'
+    const _Float16 col_seg[3] = {hi, hi, lo};
+    const _Float16 row_seg[3] = {his, los, his};
+    cv = __builtin_bit_cast(uint16_t, col_seg[seg]);
+    rv = __builtin_bit_cast(uint16_t, row_seg[seg]);
+  }
+  colside[idx] = cv;
+  rowside[idx] = rv;
 }
 
 template <class TE>
@@ -147,6 +178,7 @@ struct GramOps<LAKER_PREC_FP32> {
 // BF16 MFMA 32x32x16 on the hi/lo packed operands: chunks = ka/16.
 template <>
 struct GramOps<LAKER_PREC_MIXED> {
+  using vec = bf16x8;
   template <int NS>
   struct Operand {
     bf16x8 v[NS];
@@ -178,6 +210,38 @@ struct GramOps<LAKER_PREC_MIXED> {
     floatx16 acc = {0};
 #pragma unroll
     for (int q = 0; q < NS; ++q) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.v[q], b.v[q], acc, 0, 0, 0);
+    return acc;
+  }
+};
+
+// FP16 MFMA 32x32x16 on the 2-way FP16 split operands (square operator,
+// k = 1): chunks = ka/16.  Internal Gram mode, not a public precision.
+constexpr int GRAM_F16S = 16;
+template <>
+struct GramOps<GRAM_F16S> {
+  using vec = halfx8;
+  template <int NS>
+  struct Operand {
+    halfx8 v[NS];
+  };
+  template <int NS>
+  __device__ static inline void load(Operand<NS>& op, const void* base, int64_t row, int ka, int h) {
+    const uint16_t* p = reinterpret_cast<const uint16_t*>(base) + row * ka + h * 8;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+      if (q < ka / 16) {
+        shortx8 raw = *reinterpret_cast<const shortx8*>(p + q * 16);
+        op.v[q] = __builtin_bit_cast(halfx8, raw);
+      } else {
+        op.v[q] = __builtin_bit_cast(halfx8, shortx8{0, 0, 0, 0, 0, 0, 0, 0});
+      }
+    }
+  }
+  template <int NS>
+  __device__ static inline floatx16 gram_exact(const Operand<NS>& a, const Operand<NS>& b) {
+    floatx16 acc = {0};
+#pragma unroll
+    for (int q = 0; q < NS; ++q) acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a.v[q], b.v[q], acc, 0, 0, 0);
     return acc;
   }
 };
@@ -420,7 +484,7 @@ __global__ __launch_bounds__(256) void mv_vec1_kernel(
         } else {
 #pragma unroll
           for (int q = 0; q < NS; ++q)
-            A.v[q] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const shortx8*>(rowp + q * 32 + h * 16));
+            A.v[q] = __builtin_bit_cast(typename G::vec, *reinterpret_cast<const shortx8*>(rowp + q * 32 + h * 16));
         }
         float xv[16];
 #pragma unroll
@@ -526,8 +590,14 @@ __device__ inline float transpose_reduce16(const float (&q)[16], int lane) {
 
 __host__ __device__ inline int64_t sym_coff(int64_t b, int64_t n) { return b * n - (SYM_RB / 2) * b * (b + 1); }
 
+// Waves per SIMD the register allocation must allow: the FP16-split
+// instance with at most two Gram steps fits 168 VGPRs without spills and runs
+// three waves per SIMD instead of two; the others keep the compiler's choice.
+template <int MODE, int NS>
+constexpr int sym1_min_waves() { return MODE == GRAM_F16S && NS <= 2 ? 3 : 1; }
+
 template <int MODE, int NS, bool CLAMP>
-__global__ __launch_bounds__(256) void mv_sym1_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sym1_min_waves<MODE, NS>()))) void mv_sym1_kernel(
     const void* __restrict__ colpack, const void* __restrict__ rowpack, int64_t n,
     const float* __restrict__ X, int64_t ldx, const int2* __restrict__ map, double* __restrict__ part_r,
     double* __restrict__ part_c, float clampv, const int* __restrict__ skip) {
@@ -640,7 +710,7 @@ __global__ __launch_bounds__(256) void mv_sym1_kernel(
         } else {
 #pragma unroll
           for (int q = 0; q < NS; ++q)
-            A.v[q] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const shortx8*>(rowp + q * 32 + h * 16));
+            A.v[q] = __builtin_bit_cast(typename G::vec, *reinterpret_cast<const shortx8*>(rowp + q * 32 + h * 16));
         }
         float xv[16];
 #pragma unroll
@@ -1559,6 +1629,7 @@ static int launch_sym1(Operator& op, int mode, const void* colpack, const void* 
                        const float* X, int64_t ldx, float* Y, int64_t ldy, double lam, const float* Xr,
                        int64_t ldxr, bool clamp, float clampv, int accumulate, const int* skip,
                        hipStream_t st) {
+  // the 16-bit modes (BF16 3-way split, FP16 2-way split) share one layout
   const size_t chunk = mode == LAKER_PREC_FP32 ? V1Layout<LAKER_PREC_FP32>::chunk_bytes(ka)
                                                : V1Layout<LAKER_PREC_MIXED>::chunk_bytes(ka);
   const size_t lds = 2 * chunk + 2 * 4 * V1_JT * 32 * sizeof(float);
@@ -1593,6 +1664,14 @@ static int launch_sym1(Operator& op, int mode, const void* colpack, const void* 
       case 5: LAKER_S1(LAKER_PREC_FP32, 5); break;
       case 8: LAKER_S1(LAKER_PREC_FP32, 8); break;
       default: LAKER_S1(LAKER_PREC_FP32, 16); break;
+    }
+  } else if (mode == GRAM_F16S) {
+    switch (ns) {
+      case 1: LAKER_S1(GRAM_F16S, 1); break;
+      case 2: LAKER_S1(GRAM_F16S, 2); break;
+      case 3: LAKER_S1(GRAM_F16S, 3); break;
+      case 4: LAKER_S1(GRAM_F16S, 4); break;
+      default: LAKER_S1(GRAM_F16S, 6); break;
     }
   } else {
     switch (ns) {
@@ -1717,8 +1796,13 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
     const bool self_sq = n_rows == n_cols && rows.pack16 == op.self.row16.ptr && cols.pack16 == op.self.col16.ptr &&
                          rows.pack32 == op.self.row32.ptr && cols.pack32 == op.self.col32.ptr;
     if (k == 1 && !deriv && self_sq && op.sym_enabled) {
-      const int rs = launch_sym1(op, gmode, colpack, rowpack, ka, n_rows, X, ldx, Y, ldy, lam, Xr, ldxr, clamp,
-                                 clampv, accumulate, skip, st);
+      // split precisions: the FP16 2-way split (3 products) when the scaled
+      // embedding fits FP16, else the BF16 3-way split (6 products)
+      const bool f16s = mixed && op.f16s_ok;
+      const int rs = f16s ? launch_sym1(op, GRAM_F16S, op.self.col16h.ptr, op.self.row16h.ptr, op.ka16h, n_rows, X,
+                                        ldx, Y, ldy, lam, Xr, ldxr, clamp, clampv, accumulate, skip, st)
+                          : launch_sym1(op, gmode, colpack, rowpack, ka, n_rows, X, ldx, Y, ldy, lam, Xr, ldxr,
+                                        clamp, clampv, accumulate, skip, st);
       if (rs <= 0) return rs;
     }
     float* part = op.part.as<float>();
@@ -1995,6 +2079,24 @@ static int pack_store_t(SideStore& s, const TE* E, int64_t n, int d, int ka32, i
   return LAKER_OK;
 }
 
+// FP16 2-way split operands of the square operator (k = 1 products).
+template <class TE>
+static int pack_f16_split_t(SideStore& s, const TE* E, int64_t n, int d, int ka, double cscale, hipStream_t st) {
+  const int64_t nh = round_up(n, 64) * ka;
+  LAKER_TRY(s.col16h.ensure(nh * 2));
+  LAKER_TRY(s.row16h.ensure(nh * 2));
+  hipLaunchKernelGGL((pack_f16x2_kernel<TE>), dim3((unsigned)ceil_div(nh, 256)), dim3(256), 0, st, E, n, d,
+                     round_up(n, 64), ka, (float)cscale, s.col16h.as<uint16_t>(), s.row16h.as<uint16_t>());
+  LAKER_KCHECK("pack_f16x2_kernel");
+  return LAKER_OK;
+}
+
+int pack_f16_split(SideStore& s, const void* E, int dtype, int64_t n, int d, int ka, double scale, hipStream_t st) {
+  const double cscale = scale * kLog2eD;
+  if (dtype == LAKER_F32) return pack_f16_split_t<float>(s, (const float*)E, n, d, ka, cscale, st);
+  return pack_f16_split_t<double>(s, (const double*)E, n, d, ka, cscale, st);
+}
+
 int pack_store(SideStore& s, const void* E, int dtype, int64_t n, int d, int ka32, int ka16,
                double scale, bool need_col, bool need_row, hipStream_t st) {
   const double cscale = scale * kLog2eD;
@@ -2028,7 +2130,9 @@ int laker_operator_create(const void* E, int64_t n, int32_t d, int32_t dtype, do
   op->clamp_limit = dtype == LAKER_F32 ? 80.0 : 700.0;
   op->ka32 = (int)round_up(d, 2);
   op->ka16 = (int)round_up(6 * d, 16);
+  op->ka16h = (int)round_up(3 * d, 16);
   int rc = pack_store(op->self, E, dtype, n, d, op->ka32, op->ka16, scale, true, true, stream);
+  if (rc == LAKER_OK) rc = pack_f16_split(op->self, E, dtype, n, d, op->ka16h, scale, stream);
   if (rc != LAKER_OK) {
     delete op;
     return rc;
@@ -2089,6 +2193,8 @@ int laker_operator_create(const void* E, int64_t n, int32_t d, int32_t dtype, do
   }
   op->max_sq_norm = __builtin_bit_cast(double, bits);
   op->skip_clamp = (op->max_sq_norm * scale) < op->clamp_limit;
+  // max |e_ij| <= max ||e_i||: both split sides stay below 2^14, far from FP16 overflow
+  op->f16s_ok = std::sqrt(op->max_sq_norm) * std::max(1.0, std::fabs(scale * kLog2eD)) < 16384.0;
   *out = reinterpret_cast<laker_operator_t>(op);
   return LAKER_OK;
 }

@@ -3,8 +3,8 @@
 ``LAKERCore`` keeps the reference's hyperparameters and method signatures:
 ``compute_embeddings`` -> ``build_kernel_operator`` -> ``build_preconditioner``
 -> ``solve_pcg`` -> ``predict`` / ``predict_variance`` / ``predict_train``.
-Extra hyperparameters: ``precision`` (``"fp32"`` plain PCG, ``"mixed"`` BF16
-Gram inner PCG + FP64 refinement), ``probe_source`` and ``probe_precision``
+Extra hyperparameters: ``precision`` (``"fp32"`` plain PCG, ``"mixed"`` split-Gram
+inner PCG on 16-bit MFMA + FP64 refinement), ``probe_source`` and ``probe_precision``
 (CCCP probes), ``inner_tol``.
 """
 

@@ -112,7 +112,8 @@ class AttentionKernelOperator:
         device: device on which results are returned (default: the input's).
         dtype: ``torch.float32`` (default for fp32 inputs) or ``torch.float64``.
         precision: matvec arithmetic for float32 operators: ``"fp32"`` (FP32
-            MFMA Gram) or ``"mixed"`` (BF16 hi/lo-split MFMA Gram).
+            MFMA Gram) or ``"mixed"`` (split MFMA Gram: 3-way BF16, or 2-way
+            FP16 for the square operator's k=1 product when E fits FP16).
         scale: optional factor s in ``exp(s E E^T)`` (reference: 1.0).
     """
 

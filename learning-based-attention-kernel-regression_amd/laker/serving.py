@@ -67,9 +67,9 @@ class SolveResult:
 def footprint_bytes(n: int, k: int, d: int, nr: Optional[int] = None) -> int:
     """Peak device bytes of one request (build peak dominates)."""
     nr = nr if nr is not None else min(n, max(200, int(2 * math.sqrt(n))))
-    ka32, ka16 = 2 * ((d + 1) // 2), 16 * ((6 * d + 15) // 16)
+    ka32, ka16, ka16h = 2 * ((d + 1) // 2), 16 * ((6 * d + 15) // 16), 16 * ((3 * d + 15) // 16)
     npad = 64 * ((n + 63) // 64)
-    packs = npad * (2 * ka32 * 4 + 2 * ka16 * 2) + n * d * 12
+    packs = npad * (2 * ka32 * 4 + 2 * ka16 * 2 + 2 * ka16h * 2) + n * d * 12
     build = n * nr * (4 + 4 + 8 + 8) + nr * nr * 8 * 12
     solve = n * k * (6 * 4 + 3 * 8) + 64 * n * max(k, 4) * 4
     # partial sums of the symmetric k=1 product (column partials ~n^2/512

@@ -9,7 +9,7 @@ When ``operator`` is the bound ``matvec`` of an :class:`AttentionKernelOperator`
 and ``preconditioner`` is the bound ``apply`` of a :class:`CCCPPreconditioner`
 (or ``None``), the whole loop runs on device in the HIP library
 (``laker_pcg_solve``): vectors resident, FP64 scalar recurrences, no host sync
-per iteration.  ``precision="mixed"`` selects the BF16-Gram inner PCG with
+per iteration.  ``precision="mixed"`` selects the split-Gram inner PCG with
 FP64 iterative refinement (``laker_solve_mixed``); its stopping test is the
 FP64 true residual.  Any other callables take the generic loop, which runs on
 the GPU with the library's FP64 column dot products.

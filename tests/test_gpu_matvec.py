@@ -2,7 +2,9 @@
 
 Error metric: |y - y_ref| / (lambda |x| + |G| |x|) row-wise (scale of the
 summands), so cancellation in the sum does not inflate the tolerance.
-Tolerances: FP32 Gram 2e-5, BF16 hi/lo Gram ("mixed") 3e-4, FP64 1e-12.
+Tolerances: FP32 Gram 2e-5, split Gram ("mixed": FP16 2-way for the square
+k=1 product, BF16 3-way otherwise) 2e-5, hi/lo right-hand sides 2e-4, BF16
+right-hand sides 1e-2, FP64 1e-12 or 1e-13.
 """
 
 import numpy as np
@@ -40,7 +42,8 @@ def make(n, d, k, seed, amp=0.6, dtype=np.float32):
 
 CASES = [(1, 1, 1), (5, 3, 1), (31, 10, 1), (64, 10, 2), (100, 6, 3), (257, 16, 4), (300, 10, 5),
          (513, 16, 31), (700, 10, 64), (1000, 32, 70), (2000, 10, 1), (2000, 16, 1), (5000, 10, 1),
-         (9000, 16, 1), (600, 4, 97), (800, 16, 130), (1500, 10, 200)]
+         (9000, 16, 1), (600, 4, 97), (800, 16, 130), (1500, 10, 200), (400, 5, 1), (777, 20, 1),
+         (1000, 32, 1)]
 
 
 @pytest.mark.parametrize("n,d,k", CASES)
@@ -87,6 +90,25 @@ def test_clamp_regime(cuda, golden):
     y = op.matvec(torch.from_numpy(x).to(cuda)).cpu().numpy()
     yref = O.matvec_fp64(e, 0.05, x, clamp=80.0)
     assert norm_err(y, yref, scale_rows(e, 0.05, x, clamp=80.0)) < 2e-5
+
+
+@pytest.mark.parametrize("big", [2000.0, 5.0e4])
+def test_square_product_split_range(cuda, big):
+    """The mixed square k=1 product takes its FP16 2-way split only while the
+    scaled embedding fits FP16 (max |c e| < 2^14 with c = scale log2(e)): one row
+    e_0 = (big, 0, ..., 0) sets the range, every other row has a zero first
+    coordinate, so G[0, j] = 0 exactly and G[0, 0] = big^2 is clamped.  At
+    big = 5e4, c e_0 overflows FP16: a wrong choice of split shows up as NaN."""
+    e, x = make(600, 10, 1, seed=31)
+    e[:, 0] = 0.0
+    e[0] = 0.0
+    e[0, 0] = big
+    op = laker.AttentionKernelOperator(torch.from_numpy(e).to(cuda), lambda_reg=0.05, precision="mixed")
+    assert op.skip_clamp is False
+    y = op.matvec(torch.from_numpy(x).to(cuda)).cpu().numpy()
+    assert np.isfinite(y).all()
+    yref = O.matvec_fp64(e, 0.05, x, clamp=80.0)
+    assert norm_err(y, yref, scale_rows(e, 0.05, x, clamp=80.0)) < TOL["mixed"]
 
 
 def test_cpu_inputs_roundtrip_device(cuda):
