@@ -974,14 +974,16 @@ __host__ __device__ constexpr int m2_chunk_bytes(int ns, int nparts, int ncc, in
 
 // Column operand rows and packed X operands of JT column tiles are staged
 // global -> registers -> LDS one chunk ahead (double buffer, one barrier per
-// chunk) and shared by the block's 4 waves.
-template <int NS, int GX, int NCC, int JT, int ITW, bool CLAMP>
+// chunk) and shared by the block's 4 waves.  GM: Gram operands, the BF16
+// 3-way split (LAKER_PREC_MIXED) or the FP16 2-way split (GRAM_F16S, square
+// operator only).
+template <int GM, int NS, int GX, int NCC, int JT, int ITW, bool CLAMP>
 __global__ __launch_bounds__(256) void mv_multi2_kernel(
     const void* __restrict__ colpack, const void* __restrict__ rowpack, int ka, int64_t n_rows,
     int64_t n_cols, const bf16x8* __restrict__ xh, const bf16x8* __restrict__ xl,
     const bf16x8* __restrict__ xm, int kcols, int c_base, float* __restrict__ part, int64_t ldp,
     int jsplit, float clampv) {
-  using G = GramOps<LAKER_PREC_MIXED>;
+  using G = GramOps<GM>;
   constexpr int NP = M2Parts<GX>::value;
   constexpr int gb = 32 * NS, lb = gb + 16, vpr = gb / 16;
   constexpr int VA = JT * 32 * vpr;           // operand-row vectors per chunk
@@ -996,7 +998,7 @@ __global__ __launch_bounds__(256) void mv_multi2_kernel(
   const int64_t I0 = (ib * 4 + w) * (32 * ITW);
   const bool active = I0 < n_rows;
   (void)ka;
-  G::Operand<NS> B[ITW];
+  typename G::template Operand<NS> B[ITW];
   if (active) {
 #pragma unroll
     for (int it = 0; it < ITW; ++it) G::load(B[it], rowpack, I0 + it * 32 + li, NS * 16, h);
@@ -1053,11 +1055,11 @@ __global__ __launch_bounds__(256) void mv_multi2_kernel(
       const char* cur = m2_smem + (odd ? CHUNK : 0);
       const uint4* xs = reinterpret_cast<const uint4*>(cur + JT * 32 * lb);
       for (int tt = 0; tt < nt; ++tt) {
-        G::Operand<NS> A;
+        typename G::template Operand<NS> A;
         const char* rowp = cur + (tt * 32 + li) * lb;
 #pragma unroll
         for (int q = 0; q < NS; ++q)
-          A.v[q] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const shortx8*>(rowp + q * 32 + h * 16));
+          A.v[q] = __builtin_bit_cast(typename G::vec, *reinterpret_cast<const shortx8*>(rowp + q * 32 + h * 16));
         bf16x8 Xh[NCC][2], Xl[NCC][2], Xm[NCC][2];
 #pragma unroll
         for (int q = 0; q < NCC; ++q)
@@ -1070,7 +1072,7 @@ __global__ __launch_bounds__(256) void mv_multi2_kernel(
           }
         floatx16 D[ITW];
 #pragma unroll
-        for (int it = 0; it < ITW; ++it) D[it] = G::gram_exact<NS>(A, B[it]);
+        for (int it = 0; it < ITW; ++it) D[it] = G::template gram_exact<NS>(A, B[it]);
 #pragma unroll
         for (int it = 0; it < ITW; ++it) {
           // K ordering of chunk u: k = 8h + v  <->  j = jj(8u + v, h)
@@ -1843,8 +1845,13 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
   const int64_t kp = round_up(k, 32);
   LAKER_TRY(op.ensure_part((size_t)jsplit * n_rows * kp * sizeof(float)));
   float* part = op.part.as<float>();
+  // the square operator's Gram from the FP16 2-way split when E fits FP16 (d <= 16)
+  const bool self_sq = n_rows == n_cols && rows.pack16 == op.self.row16.ptr && cols.pack16 == op.self.col16.ptr;
+  const int ns16h = op.ka16h / 16;
+  const bool f16s = mixed && !deriv && gx != LAKER_PREC_FP32 && self_sq && op.f16s_ok && (ns16h == 2 || ns16h == 3);
   const int ns16 = op.ka16 / 16;
-  const bool packed_x = mixed && !deriv && gx != LAKER_PREC_FP32 && (ns16 == 2 || ns16 == 4 || ns16 == 6);
+  const bool packed_x =
+      mixed && !deriv && gx != LAKER_PREC_FP32 && (f16s || ns16 == 2 || ns16 == 4 || ns16 == 6);
   if (packed_x) {
     // X parts packed per pass: 3 x 2 chunks (3-way split) or 2 x 4 chunks (hi/lo, BF16)
     const int64_t ntiles = ceil_div(n_cols, 32);
@@ -1870,14 +1877,25 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
     /* one row tile per wave for the register-heavy 3-way split and 4-chunk passes */                \
     constexpr int ITWV = (GXM == GX_SPLIT3 || NC == 4) ? 1 : 2;                                       \
     dim3 grid2((unsigned)(ceil_div(n_rows, 128 * ITWV) * jsplit2));                                   \
-    if (clamp)                                                                                        \
-      hipLaunchKernelGGL((mv_multi2_kernel<NS, GXM, NC, JTV, ITWV, true>), grid2, block, lds2, st,     \
-                         colpack, rowpack, ka, n_rows, n_cols, (const bf16x8*)xh, (const bf16x8*)xl,  \
-                         (const bf16x8*)xm, k, c0, part, kp, jsplit2, clampv);                        \
+    if (f16s) {                                                                                       \
+      if (clamp)                                                                                      \
+        hipLaunchKernelGGL((mv_multi2_kernel<GRAM_F16S, NS, GXM, NC, JTV, ITWV, true>), grid2, block,  \
+                           lds2, st, op.self.col16h.ptr, op.self.row16h.ptr, op.ka16h, n_rows, n_cols, \
+                           (const bf16x8*)xh, (const bf16x8*)xl, (const bf16x8*)xm, k, c0, part, kp,   \
+                           jsplit2, clampv);                                                          \
+      else                                                                                            \
+        hipLaunchKernelGGL((mv_multi2_kernel<GRAM_F16S, NS, GXM, NC, JTV, ITWV, false>), grid2, block, \
+                           lds2, st, op.self.col16h.ptr, op.self.row16h.ptr, op.ka16h, n_rows, n_cols, \
+                           (const bf16x8*)xh, (const bf16x8*)xl, (const bf16x8*)xm, k, c0, part, kp,   \
+                           jsplit2, clampv);                                                          \
+    } else if (clamp)                                                                                 \
+      hipLaunchKernelGGL((mv_multi2_kernel<LAKER_PREC_MIXED, NS, GXM, NC, JTV, ITWV, true>), grid2,    \
+                         block, lds2, st, colpack, rowpack, ka, n_rows, n_cols, (const bf16x8*)xh,    \
+                         (const bf16x8*)xl, (const bf16x8*)xm, k, c0, part, kp, jsplit2, clampv);     \
     else                                                                                              \
-      hipLaunchKernelGGL((mv_multi2_kernel<NS, GXM, NC, JTV, ITWV, false>), grid2, block, lds2, st,    \
-                         colpack, rowpack, ka, n_rows, n_cols, (const bf16x8*)xh, (const bf16x8*)xl,  \
-                         (const bf16x8*)xm, k, c0, part, kp, jsplit2, clampv);                        \
+      hipLaunchKernelGGL((mv_multi2_kernel<LAKER_PREC_MIXED, NS, GXM, NC, JTV, ITWV, false>), grid2,   \
+                         block, lds2, st, colpack, rowpack, ka, n_rows, n_cols, (const bf16x8*)xh,    \
+                         (const bf16x8*)xl, (const bf16x8*)xm, k, c0, part, kp, jsplit2, clampv);     \
   } while (0)
 #define LAKER_M2C(NS, GXM, NC)                                                                        \
   do {                                                                                                \
@@ -1897,16 +1915,20 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
   } while (0)
 #define LAKER_M2_NS(GXM)                     \
   do {                                       \
-    if (ns16 == 2) LAKER_M2(2, GXM);         \
-    else if (ns16 == 4) LAKER_M2(4, GXM);    \
+    if (gns == 2) LAKER_M2(2, GXM);          \
+    else if (gns == 3) LAKER_M2(3, GXM);     \
+    else if (gns == 4) LAKER_M2(4, GXM);     \
     else LAKER_M2(6, GXM);                   \
   } while (0)
 #define LAKER_M2_NS3(GXM)                    \
   do {                                       \
-    if (ns16 == 2) LAKER_M2_2(2, GXM);       \
-    else if (ns16 == 4) LAKER_M2_2(4, GXM);  \
+    if (gns == 2) LAKER_M2_2(2, GXM);        \
+    else if (gns == 3) LAKER_M2_2(3, GXM);   \
+    else if (gns == 4) LAKER_M2_2(4, GXM);   \
     else LAKER_M2_2(6, GXM);                 \
   } while (0)
+      // Gram steps: ka16h / 16 (FP16 split) or ka16 / 16 (BF16 split)
+      const int gns = f16s ? ns16h : ns16;
       if (gx == GX_SPLIT3) LAKER_M2_NS3(GX_SPLIT3);
       else if (gx == LAKER_PREC_MIXED) LAKER_M2_NS(LAKER_PREC_MIXED);
       else LAKER_M2_NS(LAKER_PREC_BF16);
