@@ -545,7 +545,8 @@ __global__ __launch_bounds__(256) void mv_vec1_kernel(
 #ifndef LAKER_SYM_ITW
 #define LAKER_SYM_ITW 4
 #endif
-constexpr int SYM_SEG = 64;                // column tiles per workgroup
+constexpr int SYM_SEG = 32;                // column tiles per workgroup (1024 columns: 16..64 measured,
+                                           // 32 best for load balance against row-partial traffic)
 constexpr int SYM_ITW = LAKER_SYM_ITW;     // row tiles per wave
 constexpr int SYM_RB = 4 * 32 * SYM_ITW;   // rows per workgroup
 constexpr int SYM_DT = SYM_RB / 32;        // column tiles of the diagonal block

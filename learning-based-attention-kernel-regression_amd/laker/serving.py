@@ -73,10 +73,10 @@ def footprint_bytes(n: int, k: int, d: int, nr: Optional[int] = None) -> int:
     build = n * nr * (4 + 4 + 8 + 8) + nr * nr * 8 * 12
     solve = n * k * (6 * 4 + 3 * 8) + 64 * n * max(k, 4) * 4
     # partial sums of the symmetric k=1 product (column partials ~n^2/512
-    # doubles, row partials per 2048-column segment) and the packed BF16
+    # doubles, row partials per 1024-column segment) and the packed BF16
     # right-hand sides of the multi-RHS product (3 parts x 512 16-byte vectors
     # per 32-row tile)
-    sym = (n * n // 512 + (n // 2048 + 1) * n) * 8
+    sym = (n * n // 512 + (n // 1024 + 1) * n) * 8
     xpack = ((n + 31) // 32) * 3 * 512 * 16
     return int(packs + build + solve + sym + xpack + (64 << 20))
 
