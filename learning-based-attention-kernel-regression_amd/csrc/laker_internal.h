@@ -84,10 +84,14 @@ int pack_store(SideStore& s, const void* E, int dtype, int64_t n, int d, int ka3
 // skip: optional device flag; when it reads non-zero at launch time the k <= 4
 // kernels return at once and Y is left untouched (a finished solve's trailing
 // launches cost nothing).
+// dots3 (optional, k = 1): when the square operator's symmetric kernel runs,
+// its reduction also writes the per-256-row-block partials of Xr.Y, Xr.Xr and
+// Y.Y to dots3[q * ceil(n / 256) + block] and sets *dots3_done.
 int fused_product_f32(Operator& op, int precision, const Side& rows, const Side& cols,
                       const float* X, int k, int64_t ldx, float* Y, int64_t ldy, double lam,
                       const float* Xr, int64_t ldxr, bool clamp, bool deriv, int accumulate,
-                      hipStream_t st, const int* skip = nullptr);
+                      hipStream_t st, const int* skip = nullptr, double* dots3 = nullptr,
+                      bool* dots3_done = nullptr);
 
 // FP64 product; when B != nullptr writes the residual Y = B - (lam Xr + G X).
 template <class XT>

@@ -777,38 +777,58 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sym1_min_wa
 }
 
 // y[i] = lam xr[i] (+ y[i]) + sum_seg part_r[seg][i] + sum_{b < i/SYM_RB} part_c[coff(b) + i - SYM_RB (b+1)]
+// dots (optional, block = 256 rows): dots[q * gridDim.x + block] = the
+// block's sum of x.y (q = 0), x.x (q = 1) and y.y (q = 2) over its rows, with
+// x = Xr and y the stored Y, in the order and with the block reduction of the
+// PCG's column-dot kernel for 256-row blocks (bit-identical partials).
 __global__ void mv_sym_reduce_kernel(const double* __restrict__ part_r, const double* __restrict__ part_c,
                                      int64_t n, double lam, const float* __restrict__ Xr, int64_t ldxr,
                                      float* __restrict__ Y, int64_t ldy, int accumulate,
-                                     const int* __restrict__ skip) {
-  if (skip && *skip) return;
+                                     const int* __restrict__ skip, double* __restrict__ dots) {
+  if (skip && *skip) return;  // uniform over the grid
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int64_t ntiles = (n + 31) / 32;
-  const int64_t b = i / SYM_RB;
-  const int64_t nseg = (ntiles + SYM_SEG - 1) / SYM_SEG;
-  // eight independent chains (fixed assignment) keep the loads in flight
-  double a8[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  double tail = 0.0;
-  int64_t sg = (SYM_DT * b) / SYM_SEG;  // first segment of row block b
-  for (; sg + 8 <= nseg; sg += 8) {
+  double xd = 0.0, yd = 0.0;
+  if (i < n) {
+    const int64_t ntiles = (n + 31) / 32;
+    const int64_t b = i / SYM_RB;
+    const int64_t nseg = (ntiles + SYM_SEG - 1) / SYM_SEG;
+    // eight independent chains (fixed assignment) keep the loads in flight
+    double a8[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double tail = 0.0;
+    int64_t sg = (SYM_DT * b) / SYM_SEG;  // first segment of row block b
+    for (; sg + 8 <= nseg; sg += 8) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) a8[u] += part_r[(sg + u) * n + i];
-  }
-  for (; sg < nseg; ++sg) tail += part_r[sg * n + i];
-  int64_t bb = 0;
-  for (; bb + 8 <= b; bb += 8) {
+      for (int u = 0; u < 8; ++u) a8[u] += part_r[(sg + u) * n + i];
+    }
+    for (; sg < nseg; ++sg) tail += part_r[sg * n + i];
+    int64_t bb = 0;
+    for (; bb + 8 <= b; bb += 8) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int64_t q = bb + u;
-      a8[u] += part_c[sym_coff(q, n) + i - SYM_RB * (q + 1)];
+      for (int u = 0; u < 8; ++u) {
+        const int64_t q = bb + u;
+        a8[u] += part_c[sym_coff(q, n) + i - SYM_RB * (q + 1)];
+      }
+    }
+    for (; bb < b; ++bb) tail += part_c[sym_coff(bb, n) + i - SYM_RB * (bb + 1)];
+    double s = (((a8[0] + a8[1]) + (a8[2] + a8[3])) + ((a8[4] + a8[5]) + (a8[6] + a8[7]))) + tail;
+    if (Xr) s += lam * (double)Xr[i * ldxr];
+    if (accumulate) s += (double)Y[i * ldy];
+    const float y = (float)s;
+    Y[i * ldy] = y;
+    if (dots) {
+      xd = (double)Xr[i * ldxr];
+      yd = (double)y;
     }
   }
-  for (; bb < b; ++bb) tail += part_c[sym_coff(bb, n) + i - SYM_RB * (bb + 1)];
-  double s = (((a8[0] + a8[1]) + (a8[2] + a8[3])) + ((a8[4] + a8[5]) + (a8[6] + a8[7]))) + tail;
-  if (Xr) s += lam * (double)Xr[i * ldxr];
-  if (accumulate) s += (double)Y[i * ldy];
-  Y[i * ldy] = (float)s;
+  if (dots) {  // uniform: every thread of the block takes part
+    __shared__ double red[16];
+    const double v[3] = {xd * yd, xd * xd, yd * yd};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double t = block_sum(v[q], red);
+      if (threadIdx.x == 0) dots[(int64_t)q * gridDim.x + blockIdx.x] = t;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1631,7 +1651,7 @@ static int launch_vec1(int mode, const void* colpack, const void* rowpack, int k
 static int launch_sym1(Operator& op, int mode, const void* colpack, const void* rowpack, int ka, int64_t n,
                        const float* X, int64_t ldx, float* Y, int64_t ldy, double lam, const float* Xr,
                        int64_t ldxr, bool clamp, float clampv, int accumulate, const int* skip,
-                       hipStream_t st) {
+                       hipStream_t st, double* dots) {
   // the 16-bit modes (BF16 3-way split, FP16 2-way split) share one layout
   const size_t chunk = mode == LAKER_PREC_FP32 ? V1Layout<LAKER_PREC_FP32>::chunk_bytes(ka)
                                                : V1Layout<LAKER_PREC_MIXED>::chunk_bytes(ka);
@@ -1688,7 +1708,8 @@ static int launch_sym1(Operator& op, int mode, const void* colpack, const void* 
 #undef LAKER_S1
   LAKER_KCHECK("mv_sym1_kernel");
   hipLaunchKernelGGL(mv_sym_reduce_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st,
-                     (const double*)part_r, (const double*)part_c, n, lam, Xr, ldxr, Y, ldy, accumulate, skip);
+                     (const double*)part_r, (const double*)part_c, n, lam, Xr, ldxr, Y, ldy, accumulate, skip,
+                     Xr ? dots : nullptr);
   LAKER_KCHECK("mv_sym_reduce_kernel");
   return LAKER_OK;
 }
@@ -1783,7 +1804,8 @@ static int launch_f64(const double* Ecol, const double* Erow, int d, int64_t n_r
 int fused_product_f32(Operator& op, int precision, const Side& rows, const Side& cols,
                       const float* X, int k, int64_t ldx, float* Y, int64_t ldy, double lam,
                       const float* Xr, int64_t ldxr, bool clamp, bool deriv, int accumulate,
-                      hipStream_t st, const int* skip) {
+                      hipStream_t st, const int* skip, double* dots3, bool* dots3_done) {
+  if (dots3_done) *dots3_done = false;
   LAKER_CHECK_ARG(k >= 1, "k must be >= 1");
   const float clampv = (float)(op.clamp_limit * kLog2eD);
   const bool mixed = precision == LAKER_PREC_MIXED || precision == LAKER_PREC_BF16 || precision == LAKER_PREC_HILO;
@@ -1802,10 +1824,12 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
       // split precisions: the FP16 2-way split (3 products) when the scaled
       // embedding fits FP16, else the BF16 3-way split (6 products)
       const bool f16s = mixed && op.f16s_ok;
+      double* d3 = (dots3 && Xr) ? dots3 : nullptr;
       const int rs = f16s ? launch_sym1(op, GRAM_F16S, op.self.col16h.ptr, op.self.row16h.ptr, op.ka16h, n_rows, X,
-                                        ldx, Y, ldy, lam, Xr, ldxr, clamp, clampv, accumulate, skip, st)
+                                        ldx, Y, ldy, lam, Xr, ldxr, clamp, clampv, accumulate, skip, st, d3)
                           : launch_sym1(op, gmode, colpack, rowpack, ka, n_rows, X, ldx, Y, ldy, lam, Xr, ldxr,
-                                        clamp, clampv, accumulate, skip, st);
+                                        clamp, clampv, accumulate, skip, st, d3);
+      if (rs == 0 && d3 && dots3_done) *dots3_done = true;
       if (rs <= 0) return rs;
     }
     float* part = op.part.as<float>();

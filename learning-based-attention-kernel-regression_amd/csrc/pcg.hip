@@ -173,6 +173,64 @@ __global__ __launch_bounds__(256) void pcg_update_xr_rr_kernel(T* __restrict__ x
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// alpha = rz / pAp and the breakdown test of pcg_alpha_kernel (k = 1),
+// evaluated by every block from the same partials in the same order, so all
+// blocks agree; on breakdown block 0 records it and no block updates.
+// Returns false when the block must not update.
+__device__ inline bool block_alpha1(const double* __restrict__ part, int nblk, const double* __restrict__ rz,
+                                    PcgState* __restrict__ st, double* alpha_out) {
+  __shared__ double sums[3], sh[3];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (w < 3) {  // wave q sums quantity q: p.Ap, p.p, Ap.Ap
+    const double v = wave_sum_part(part, w, nblk, 1, 0, lane);
+    if (lane == 0) sums[w] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double pap = sums[0], pp = sums[1], aa = sums[2];
+    sh[0] = pap;
+    sh[1] = (pap <= -st->eps * sqrt(pp) * sqrt(aa)) ? 1.0 : 0.0;
+    sh[2] = rz[0] / pap;
+  }
+  __syncthreads();
+  if (sh[1] != 0.0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->done = PCG_BREAKDOWN;
+      st->iterations = st->it;  // iterations completed before the breakdown
+    }
+    return false;
+  }
+  *alpha_out = sh[2];
+  return true;
+}
+
+// k = 1, alpha folded in (the column-dot partials of p.Ap, p.p, Ap.Ap in
+// dots[0 .. 3 nblk)): x += alpha p ; r -= alpha Ap, and rr[blk] = the block's
+// sum of r_new^2.  rr must not overlap dots (other blocks still read them).
+template <class T>
+__global__ __launch_bounds__(256) void pcg_update_xr_rr_alpha_kernel(T* __restrict__ x, T* __restrict__ r,
+                                                                     const T* __restrict__ p,
+                                                                     const T* __restrict__ ap,
+                                                                     const double* __restrict__ dots, int nblk,
+                                                                     const double* __restrict__ rz, int64_t n,
+                                                                     double* __restrict__ rr,
+                                                                     PcgState* __restrict__ st) {
+  if (st->done) return;  // written by an earlier kernel: uniform over the grid
+  double a;
+  if (!block_alpha1(dots, nblk, rz, st, &a)) return;
+  __shared__ double red[16];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double r2 = 0.0;
+  if (i < n) {
+    x[i] = (T)((double)x[i] + a * (double)p[i]);
+    const T rn = (T)((double)r[i] - a * (double)ap[i]);
+    r[i] = rn;
+    r2 = (double)rn * (double)rn;
+  }
+  const double s = block_sum(r2, red);
+  if (threadIdx.x == 0) rr[blockIdx.x] = s;
+}
+
 // residual norm, convergence and max-iteration bookkeeping.  One block.
 __global__ void pcg_resid_kernel(const double* __restrict__ part, int nblk, int k,
                                  PcgState* __restrict__ st) {
@@ -197,10 +255,59 @@ __global__ void pcg_resid_kernel(const double* __restrict__ part, int nblk, int 
   }
 }
 
+// k = 1: the bookkeeping of pcg_resid_kernel (from the ||r||^2 partials rr of
+// the update kernel; every block evaluates the same test, block 0 records it)
+// followed, unless the run just stopped, by the r.z column-dot partials of
+// coldots_kernel<T, 1> (blocks of rpb rows).  The preconditioner kernels that
+// run between the update and this kernel do not read the state.  The
+// iteration counter moves on here only when the run stops (every block must
+// read the same st->it); otherwise pcg_beta_kernel advances it.
+template <class T>
+__global__ __launch_bounds__(256) void pcg_resid_rz_kernel(const double* __restrict__ rr, int nrr,
+                                                           const T* __restrict__ r, const T* __restrict__ z,
+                                                           int64_t n, int64_t rpb, double* __restrict__ part,
+                                                           PcgState* __restrict__ st) {
+  if (st->done) return;  // written by an earlier kernel: uniform over the grid
+  __shared__ int stop;
+  const int lane = threadIdx.x & 63;
+  if (threadIdx.x < 64) {
+    const double s = wave_sum_part(rr, 0, nrr, 1, 0, lane);
+    if (lane == 0) {
+      const int it = st->it, max_iter = st->max_iter;
+      const double res = sqrt(s);
+      const bool conv = res / st->rhs_norm <= st->tol;
+      const bool last = !conv && it + 1 >= max_iter;
+      stop = (conv || last) ? 1 : 0;
+      if (blockIdx.x == 0) {
+        st->res_norm = res;
+        if (conv) {
+          st->done = PCG_CONVERGED;
+          st->iterations = it + 1;
+        } else if (last) {
+          st->done = PCG_MAXITER;
+          st->iterations = max_iter;
+        }
+        if (conv || last) st->it = it + 1;
+      }
+    }
+  }
+  __syncthreads();
+  if (stop) return;
+  const int64_t r0 = (int64_t)blockIdx.x * rpb;
+  const int64_t r1 = (r0 + rpb < n) ? r0 + rpb : n;
+  double acc = 0.0;
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) acc += (double)r[i] * (double)z[i];
+  __shared__ double red[16];
+  const double t = block_sum(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
 // beta = rz_new / rz_old ; rz_old = rz_new.  One block.
+// bump_it: also advance the iteration counter (after pcg_resid_rz_kernel).
 __global__ void pcg_beta_kernel(const double* __restrict__ part, int nblk, int k, double* __restrict__ rz,
-                                double* __restrict__ beta, const PcgState* __restrict__ st) {
+                                double* __restrict__ beta, PcgState* __restrict__ st, int bump_it) {
   if (st->done) return;
+  if (bump_it && threadIdx.x == 0) st->it += 1;
   const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
   for (int c = threadIdx.x >> 6; c < k; c += nw) {
     const double v = wave_sum_part(part, 0, nblk, k, c, lane);
@@ -294,17 +401,27 @@ static int coldots(const PcgBuffers<T>& b, int nq, const T* a0, const T* b0, con
 }
 
 // A p for the operator in its dtype and the requested precision.
+// dots3 (optional): see fused_product_f32; *dots3_done tells whether the
+// p.Ap, p.p and Ap.Ap partials were produced on the way.
 template <class T>
 static int op_apply(Operator& op, int precision, const T* X, int k, T* Y, const int* skip,
-                    hipStream_t st) {
+                    hipStream_t st, double* dots3 = nullptr, bool* dots3_done = nullptr) {
   const bool clamp = op.clamp_for(LAKER_CLAMP_AUTO);
+  if (dots3_done) *dots3_done = false;
   if constexpr (sizeof(T) == 8) {
     return fused_product_f64<double>(op, op.rows(), op.cols(), (const double*)X, k, k, (double*)Y, k,
                                      op.lambda_reg, (const double*)X, k, clamp, nullptr, st);
   } else {
     return fused_product_f32(op, precision, op.rows(), op.cols(), (const float*)X, k, k, (float*)Y, k,
-                             op.lambda_reg, (const float*)X, k, clamp, false, 0, st, skip);
+                             op.lambda_reg, (const float*)X, k, clamp, false, 0, st, skip, dots3, dots3_done);
   }
+}
+
+// The product's reduction can emit the column-dot partials itself when the
+// dot blocks are its own 256-row blocks (k = 1).
+template <class T>
+static bool dots_fusable(const PcgBuffers<T>& b, int64_t n, int k) {
+  return k == 1 && b.rpb == 256 && (int64_t)b.nblk == ceil_div(n, 256);
 }
 
 template <class T>
@@ -321,7 +438,9 @@ static int pcg_buffers(Operator& op, int k, PcgBuffers<T>& b) {
   const size_t vec_bytes = round_up(nk * sizeof(T), 256);
   // coldots partials (3 x nblk x k) or, for k = 1, the per-256-row partials of
   // the fused update kernels
-  const size_t part_bytes = round_up(std::max<size_t>((size_t)3 * nblk * k, (size_t)ceil_div(op.n, 256)) * 8, 256);
+  // (3 nblk k), then, for k = 1, the per-256-row partials of ||r||^2 written by
+  // the fused update kernels while other blocks still read the first part
+  const size_t part_bytes = round_up(((size_t)3 * nblk * k + (size_t)ceil_div(op.n, 256)) * 8, 256);
   const size_t col_bytes = round_up((size_t)k * 8, 256);
   const size_t need = 5 * vec_bytes + part_bytes + 3 * col_bytes + 256;
   LAKER_TRY(op.pcg_ws.ensure(need));
@@ -446,36 +565,47 @@ static int pcg_run(Operator& op, Precond* pc, const T* B, int k, T* X, const T* 
   }
   if (check_every < 1) check_every = 1;
   // one iteration: Ap, p.Ap (+ breakdown test), x/r update, ||r||, z = P r, beta, p
+  double* rr_part = b.part + (size_t)3 * nblk * k;  // k = 1: ||r||^2 partials
+  const bool fuse_dots = dots_fusable(b, n, k);
   auto body = [&]() -> int {
-    LAKER_TRY(op_apply<T>(op, precision, b.p, k, b.ap, &b.st->done, st));
-    LAKER_TRY(coldots<T>(b, 3, b.p, b.ap, b.p, b.p, b.ap, b.ap, n, k, true, st));
-    hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.alpha, b.st);
-    LAKER_KCHECK("pcg_alpha_kernel");
+    bool dots_done = false;
+    LAKER_TRY(op_apply<T>(op, precision, b.p, k, b.ap, &b.st->done, st, fuse_dots ? b.part : nullptr,
+                          &dots_done));
+    if (!dots_done) LAKER_TRY(coldots<T>(b, 3, b.p, b.ap, b.p, b.p, b.ap, b.ap, n, k, true, st));
     if (k == 1) {
-      // ||r||^2 partials come out of the update itself (restarted runs redo
-      // them after the residual replacement in tail)
-      hipLaunchKernelGGL((pcg_update_xr_rr_kernel<T>), eg, eb, 0, st, b.x, b.r, (const T*)b.p, (const T*)b.ap,
-                         (const double*)b.alpha, n, b.part, (const PcgState*)b.st);
-      LAKER_KCHECK("pcg_update_xr_rr_kernel");
+      // alpha and the ||r||^2 partials come out of the update itself
+      // (restarted runs redo the partials after the residual replacement in tail)
+      hipLaunchKernelGGL((pcg_update_xr_rr_alpha_kernel<T>), eg, eb, 0, st, b.x, b.r, (const T*)b.p,
+                         (const T*)b.ap, (const double*)b.part, nblk, (const double*)b.rz, n, rr_part, b.st);
+      LAKER_KCHECK("pcg_update_xr_rr_alpha_kernel");
     } else {
+      hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.alpha, b.st);
+      LAKER_KCHECK("pcg_alpha_kernel");
       hipLaunchKernelGGL((pcg_update_xr_kernel<T, T>), eg, eb, 0, st, b.x, b.r, b.p, b.ap, b.alpha, n, k,
                          b.st);
       LAKER_KCHECK("pcg_update_xr_kernel");
     }
     return LAKER_OK;
   };
+  // k = 1 without restarts: the residual test rides on the r.z dot kernel
+  const bool fuse_resid = k == 1 && restart_freq <= 0;
   auto tail = [&]() -> int {
-    if (k == 1 && restart_freq <= 0) {
-      hipLaunchKernelGGL(pcg_resid_kernel, dim3(1), dim3(256), 0, st, b.part, (int)eg.x, 1, b.st);
-    } else {
+    if (!fuse_resid) {
       LAKER_TRY(coldots<T>(b, 1, b.r, b.r, nullptr, nullptr, nullptr, nullptr, n, k, true, st));
       hipLaunchKernelGGL(pcg_resid_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.st);
+      LAKER_KCHECK("pcg_resid_kernel");
     }
-    LAKER_KCHECK("pcg_resid_kernel");
     if (pc) LAKER_TRY(precond_apply(*pc, b.r, k, k, b.z, k, st));
     else hipLaunchKernelGGL((copy_kernel<T>), eg, eb, 0, st, (const T*)b.r, b.z, nk);
-    LAKER_TRY(coldots<T>(b, 1, b.r, b.z, nullptr, nullptr, nullptr, nullptr, n, k, true, st));
-    hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.beta, b.st);
+    if (fuse_resid) {
+      hipLaunchKernelGGL((pcg_resid_rz_kernel<T>), dim3(nblk), dim3(256), 0, st, (const double*)rr_part, (int)eg.x,
+                         (const T*)b.r, (const T*)b.z, n, b.rpb, b.part, b.st);
+      LAKER_KCHECK("pcg_resid_rz_kernel");
+    } else {
+      LAKER_TRY(coldots<T>(b, 1, b.r, b.z, nullptr, nullptr, nullptr, nullptr, n, k, true, st));
+    }
+    hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.beta, b.st,
+                       fuse_resid ? 1 : 0);
     LAKER_KCHECK("pcg_beta_kernel");
     hipLaunchKernelGGL((pcg_update_p_kernel<T>), eg, eb, 0, st, b.p, b.z, b.beta, n, k, b.st);
     LAKER_KCHECK("pcg_update_p_kernel");
@@ -581,6 +711,31 @@ __global__ __launch_bounds__(256) void pcg_update_x64_rr_kernel(double* __restri
   }
   const double s = block_sum(rr, red);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// as pcg_update_x64_rr_kernel with alpha folded in (see
+// pcg_update_xr_rr_alpha_kernel); rr must not overlap dots
+__global__ __launch_bounds__(256) void pcg_update_x64_rr_alpha_kernel(double* __restrict__ x, float* __restrict__ r,
+                                                                      const float* __restrict__ p,
+                                                                      const float* __restrict__ ap,
+                                                                      const double* __restrict__ dots, int nblk,
+                                                                      const double* __restrict__ rz, int64_t n,
+                                                                      double* __restrict__ rr,
+                                                                      PcgState* __restrict__ st) {
+  if (st->done) return;  // written by an earlier kernel: uniform over the grid
+  double a;
+  if (!block_alpha1(dots, nblk, rz, st, &a)) return;
+  __shared__ double red[16];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double r2 = 0.0;
+  if (i < n) {
+    x[i] += st->xscale * a * (double)p[i];
+    const float rn = (float)((double)r[i] - a * (double)ap[i]);
+    r[i] = rn;
+    r2 = (double)rn * (double)rn;
+  }
+  const double s = block_sum(r2, red);
+  if (threadIdx.x == 0) rr[blockIdx.x] = s;
 }
 
 // x (FP64) += f * alpha p ; r -= alpha Ap  (FP32 recursion on a scaled residual)
@@ -689,28 +844,39 @@ int laker_solve_mixed(laker_operator_t oh, laker_precond_t ph, const double* B, 
   const int check_every = 8;
   PcgState hs;
   // one inner iteration; x accumulates in FP64 with the step's scale (device state)
+  double* rr_part = b.part + (size_t)3 * nblk * k;  // k = 1: ||r||^2 partials
+  const bool fuse_dots = dots_fusable(b, n, k);
   auto body = [&]() -> int {
-    LAKER_TRY(op_apply<float>(op, inner_precision, b.p, k, b.ap, &b.st->done, st));
-    LAKER_TRY(coldots<float>(b, 3, b.p, b.ap, b.p, b.p, b.ap, b.ap, n, k, true, st));
-    hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.alpha, b.st);
-    LAKER_KCHECK("pcg_alpha_kernel");
+    bool dots_done = false;
+    LAKER_TRY(op_apply<float>(op, inner_precision, b.p, k, b.ap, &b.st->done, st, fuse_dots ? b.part : nullptr,
+                              &dots_done));
+    if (!dots_done) LAKER_TRY(coldots<float>(b, 3, b.p, b.ap, b.p, b.p, b.ap, b.ap, n, k, true, st));
     if (k == 1) {
-      hipLaunchKernelGGL(pcg_update_x64_rr_kernel, eg, eb, 0, st, X, b.r, (const float*)b.p,
-                         (const float*)b.ap, (const double*)b.alpha, n, b.part, (const PcgState*)b.st);
-      LAKER_KCHECK("pcg_update_x64_rr_kernel");
-      hipLaunchKernelGGL(pcg_resid_kernel, dim3(1), dim3(256), 0, st, b.part, (int)eg.x, 1, b.st);
+      hipLaunchKernelGGL(pcg_update_x64_rr_alpha_kernel, eg, eb, 0, st, X, b.r, (const float*)b.p,
+                         (const float*)b.ap, (const double*)b.part, nblk, (const double*)b.rz, n, rr_part, b.st);
+      LAKER_KCHECK("pcg_update_x64_rr_alpha_kernel");
     } else {
+      hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.alpha, b.st);
+      LAKER_KCHECK("pcg_alpha_kernel");
       hipLaunchKernelGGL(pcg_update_x64_kernel, eg, eb, 0, st, X, b.r, (const float*)b.p, (const float*)b.ap,
                          (const double*)b.alpha, n, k, (const PcgState*)b.st);
       LAKER_KCHECK("pcg_update_x64_kernel");
       LAKER_TRY(coldots<float>(b, 1, b.r, b.r, nullptr, nullptr, nullptr, nullptr, n, k, true, st));
       hipLaunchKernelGGL(pcg_resid_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.st);
+      LAKER_KCHECK("pcg_resid_kernel");
     }
-    LAKER_KCHECK("pcg_resid_kernel");
     if (pc) LAKER_TRY(precond_apply(*pc, b.r, k, k, b.z, k, st));
     else hipLaunchKernelGGL((copy_kernel<float>), eg, eb, 0, st, (const float*)b.r, b.z, nk);
-    LAKER_TRY(coldots<float>(b, 1, b.r, b.z, nullptr, nullptr, nullptr, nullptr, n, k, true, st));
-    hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.beta, b.st);
+    if (k == 1) {
+      // the residual test of this iteration rides on the r.z dot kernel
+      hipLaunchKernelGGL((pcg_resid_rz_kernel<float>), dim3(nblk), dim3(256), 0, st, (const double*)rr_part,
+                         (int)eg.x, (const float*)b.r, (const float*)b.z, n, b.rpb, b.part, b.st);
+      LAKER_KCHECK("pcg_resid_rz_kernel");
+    } else {
+      LAKER_TRY(coldots<float>(b, 1, b.r, b.z, nullptr, nullptr, nullptr, nullptr, n, k, true, st));
+    }
+    hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.beta, b.st,
+                       k == 1 ? 1 : 0);
     LAKER_KCHECK("pcg_beta_kernel");
     hipLaunchKernelGGL((pcg_update_p_kernel<float>), eg, eb, 0, st, b.p, (const float*)b.z,
                        (const double*)b.beta, n, k, (const PcgState*)b.st);
