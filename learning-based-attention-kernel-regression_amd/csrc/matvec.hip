@@ -1352,6 +1352,32 @@ __device__ inline double row_sum16(double v) {
   return v;
 }
 
+// Transpose-reduce of four values over the 16 lanes of a DPP row: lane m ends
+// with the 16-lane sum of q[a(m)], a = {0,2,1,3,3,1,2,0,0,2,1,3,3,1,2,0}[m],
+// bit-identical to row_sum16(q[a(m)]): the levels pair the same lanes in the
+// same order (lane ^ 1, lane ^ 2, row_half_mirror, row_mirror) and every
+// addition has the same two operands, but each lane adds only the values it
+// keeps (27 instructions instead of 4 x 12).  a(m) is invariant under both
+// mirrors, and lanes m and m ^ 1 (m ^ 2) keep complementary values.
+constexpr uint32_t TR16_A = 0u | (2u << 2) | (1u << 4) | (3u << 6) | (3u << 8) | (1u << 10) | (2u << 12) |
+                            (0u << 14) | (0u << 16) | (2u << 18) | (1u << 20) | (3u << 22) | (3u << 24) |
+                            (1u << 26) | (2u << 28) | (0u << 30);
+__device__ inline int tr16_value(int m) { return (int)((TR16_A >> (2 * m)) & 3u); }
+__device__ inline double tr_sum16x4(const double (&q)[4], int am) {
+  const bool s1 = am >> 1, s2 = am & 1;
+  double t[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const double keep = s1 ? q[k + 2] : q[k], send = s1 ? q[k] : q[k + 2];
+    t[k] = keep + dpp_d<0xB1>(send);  // quad_perm [1,0,3,2]
+  }
+  const double keep = s2 ? t[1] : t[0], send = s2 ? t[0] : t[1];
+  double u = keep + dpp_d<0x4E>(send);  // quad_perm [2,3,0,1]
+  u += dpp_d<0x141>(u);                 // row_half_mirror
+  u += dpp_d<0x140>(u);                 // row_mirror
+  return u;
+}
+
 template <int STEPS, bool CLAMP, class XT>
 __global__ __launch_bounds__(256) void mv_sym64_kernel(const double* __restrict__ E, int d, int64_t n,
                                                         double rscale, double clampv,
@@ -1364,6 +1390,7 @@ __global__ __launch_bounds__(256) void mv_sym64_kernel(const double* __restrict_
   tab[threadIdx.x] = exp2((double)threadIdx.x * (1.0 / 256.0));
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int m = lane & 15, kq = lane >> 4;
+  const int am = tr16_value(m);  // value index this lane's column sum ends with
   const int2 bs = map[blockIdx.x];
   const int64_t b = bs.x;
   const int64_t I0 = b * S64_RB + w * 64;
@@ -1420,12 +1447,8 @@ __global__ __launch_bounds__(256) void mv_sym64_kernel(const double* __restrict_
             q[r] = __builtin_fma(ev, xr[it], q[r]);
           }
         }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) q[r] = row_sum16(q[r]);
-        if (m == 0) {
-#pragma unroll
-          for (int r = 0; r < 4; ++r) colbuf[w][tt * 16 + kq + 4 * r] = q[r];
-        }
+        const double cs = tr_sum16x4(q, am);
+        if (m < 4) colbuf[w][tt * 16 + kq + 4 * am] = cs;
       }
     }
     __syncthreads();
