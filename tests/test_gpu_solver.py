@@ -203,3 +203,33 @@ def test_precond_apply_multi_rhs_matches_columns(cuda, n, k):
     err = float(((z - cols).abs().max() / cols.abs().max()))
     print(f"apply n={n} k={k} N_r={pre.nr}: max rel diff vs per-column {err:.2e}", flush=True)
     assert z.shape == xs.shape and err < 2e-6
+
+
+def test_solve_beyond_dot_block_limit(cuda):
+    """n = 140,000 > 512 x 256: the PCG's column dots use 512 blocks of 274 rows,
+    so the product's reduction cannot supply the p.Ap / p.p / Ap.Ap partials
+    (its blocks are 256 rows) and the separate dot kernel runs, feeding the
+    fused alpha/update and residual-test kernels.  Both precisions must
+    converge, the mixed one to the FP64 true residual."""
+    n = 140_000
+    print(f"solve n={n} beyond the fused-dot block limit", flush=True)
+    x_loc, y = laker.pathloss_problem(n, seed=1)
+    emb = laker.PositionEmbedding(2, 10, device=cuda)
+    with torch.no_grad():
+        e = emb(x_loc.to(cuda)).contiguous()
+    yd = y.to(cuda)
+    op = laker.AttentionKernelOperator(e, lambda_reg=0.1)
+    pre = laker.CCCPPreconditioner(num_probes=128, gamma=0.1, verbose=False).build(op.matvec, n, seed=0)
+    pcg = laker.PreconditionedConjugateGradient(tol=1e-5, max_iter=2000, verbose=False)
+    alpha = pcg.solve(op.matvec, pre.apply, yd)
+    r64 = op.residual_fp64(yd.double(), alpha.double())
+    rel32 = float(torch.linalg.norm(r64) / torch.linalg.norm(yd.double()))
+    print(f"fp32: {pcg.iterations} iterations, FP64 relres {rel32:.2e}", flush=True)
+    assert 0 < pcg.iterations < 2000 and np.isfinite(rel32) and rel32 < 1e-1
+    pm = laker.PreconditionedConjugateGradient(tol=1e-6, max_iter=2000, verbose=False, precision="mixed")
+    pm.solve(op.matvec, pre.apply, yd)
+    am = pm.solution_fp64
+    rm = op.residual_fp64(yd.double(), am)
+    relm = float(torch.linalg.norm(rm) / torch.linalg.norm(yd.double()))
+    print(f"mixed: outer {pm.outer_iterations} inner {pm.iterations} FP64 relres {relm:.2e}", flush=True)
+    assert pm.converged and relm <= 1e-6
