@@ -200,24 +200,32 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double* __restrict__ A,
   const int tid = threadIdx.x, c = tid & 63, rg = tid >> 6;
   for (int i = rg; i < PNB; i += 4)
     L[i][c] = (i < nb && c < nb) ? A[(k0 + i) * lda + k0 + c] : (i == c ? 1.0 : 0.0);
+  // pivot j: sqrt of the updated diagonal entry (a non-positive one is
+  // recorded in info and replaced, so the factorisation runs on without NaNs)
+  auto pivot = [&](int j) {
+    double v = L[j][j];
+    if (!(v > 0.0)) {
+      if (*info == 0) *info = (int)(k0 + j + 1);
+      v = 1e300;
+    }
+    L[j][j] = sqrt(v);
+  };
+  if (tid == 0 && nb > 0) pivot(0);
   __syncthreads();
   for (int j = 0; j < nb; ++j) {
-    if (tid == 0) {
-      double v = L[j][j];
-      if (!(v > 0.0)) {
-        if (*info == 0) *info = (int)(k0 + j + 1);
-        v = 1e300;  // keep going without NaNs; the caller checks info
-      }
-      L[j][j] = sqrt(v);
-    }
-    __syncthreads();
     const double rjj = 1.0 / L[j][j];
     if (tid > j && tid < nb) L[tid][j] *= rjj;
     __syncthreads();
-    // trailing update of the lower triangle within the block
-    if (c > j && c < nb)
+    // trailing update of the lower triangle within the block; thread
+    // (c = j + 1, rg = 0) applies the last update to entry (j + 1, j + 1) and
+    // takes that pivot at once (no other thread reads it in this phase)
+    if (c > j && c < nb) {
+      const double lcj = L[c][j];
+#pragma unroll 4
       for (int i = j + 1 + rg; i < nb; i += 4)
-        if (c <= i) L[i][c] -= L[i][j] * L[c][j];
+        if (c <= i) L[i][c] -= L[i][j] * lcj;
+    }
+    if (tid == j + 1 && j + 1 < nb) pivot(j + 1);
     __syncthreads();
   }
   for (int i = rg; i < nb; i += 4)
