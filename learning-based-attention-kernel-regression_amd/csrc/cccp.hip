@@ -29,6 +29,10 @@ struct Precond {
   int dtype = LAKER_F32;
   double a = 1.0, rho = 0.0, inv_sqrt_a = 1.0;
   DevBuf Q, R, C, S;
+  // FP16 copy of Q and the matching S' (made on first use by the mixed solve):
+  // P' = a^{-1/2} I + Qh S' Qh^T with S' = L^{-T} S L^{-1}, L L^T = Qh^T Qh
+  DevBuf Qh, Sh;
+  bool has_h = false;
   DevBuf w1, w2, part, ws, scal;
   DevBuf inv_diag;  // Jacobi: 1 / max(diag, eps(dtype)), n entries of dtype
 };
@@ -593,6 +597,114 @@ __global__ __launch_bounds__(256) void gemv64_sum8_kernel(const double* __restri
   }
 }
 
+// Q^T x for an FP16 Q with nr % 8 == 0: as qtx4_kernel with 8-column octets
+// per 16-byte load (lane l owns octets l, l + 64, ...), FP64 accumulation.
+template <int OPL>
+__global__ __launch_bounds__(256) void qtx8h_kernel(const _Float16* __restrict__ Q, const float* __restrict__ x,
+                                                    int64_t n, int nr, int64_t rows_per_block,
+                                                    double* __restrict__ part) {
+  __shared__ double red[4][OPL * 512];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int no = nr >> 3;
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+  const int64_t r1 = (r0 + rows_per_block < n) ? r0 + rows_per_block : n;
+  double acc[OPL][8];
+#pragma unroll
+  for (int u = 0; u < OPL; ++u)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[u][j] = 0.0;
+  int64_t i = r0 + w;
+  for (; i + 12 < r1; i += 16) {
+    halfx8 q[4][OPL];
+    float xv[4];
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+      xv[rr] = x[i + 4 * rr];
+      const uint4* row = reinterpret_cast<const uint4*>(Q + (i + 4 * rr) * nr);
+#pragma unroll
+      for (int u = 0; u < OPL; ++u) {
+        const int oi = lane + 64 * u;
+        q[rr][u] = __builtin_bit_cast(halfx8, (oi < no) ? row[oi] : uint4{0u, 0u, 0u, 0u});
+      }
+    }
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr)
+#pragma unroll
+      for (int u = 0; u < OPL; ++u)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[u][j] = fma((double)q[rr][u][j], (double)xv[rr], acc[u][j]);
+  }
+  for (; i < r1; i += 4) {
+    const double xv = (double)x[i];
+    const uint4* row = reinterpret_cast<const uint4*>(Q + i * nr);
+#pragma unroll
+    for (int u = 0; u < OPL; ++u) {
+      const int oi = lane + 64 * u;
+      if (oi < no) {
+        const halfx8 q = __builtin_bit_cast(halfx8, row[oi]);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[u][j] = fma((double)q[j], xv, acc[u][j]);
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < OPL; ++u)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) red[w][(lane + 64 * u) * 8 + j] = acc[u][j];
+  __syncthreads();
+  for (int c = threadIdx.x; c < nr; c += 256)
+    part[(int64_t)blockIdx.x * nr + c] = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
+}
+
+// z[i] = f * x[i] + sum_c Q[i][c] w[c] for an FP16 Q with nr % 8 == 0 and
+// nr <= 640: sixteen lanes per row (lane 16 r + o takes octets o + 16 t,
+// t < 5, of row r), four rows per wave at a time, RPW rows per wave.  Each
+// lane keeps the w entries of its five octets in registers (loaded once per
+// wave), so the loop is loads and FMAs only; the row sum closes over the
+// sixteen lanes with four shuffles.  Octets past the row end hold zeros.
+template <int RPW>
+__global__ __launch_bounds__(256) void qw8h_kernel(const _Float16* __restrict__ Q, const double* __restrict__ w,
+                                                   const float* __restrict__ x, float* __restrict__ z, int64_t n,
+                                                   int nr, double f) {
+  constexpr int OT = 5;
+  const int lane = threadIdx.x & 63, rs = lane >> 4, ol = lane & 15;
+  const int no = nr >> 3;
+  double wr[OT][8];
+#pragma unroll
+  for (int t = 0; t < OT; ++t) {
+    const int oi = ol + 16 * t;
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) wr[t][jj] = (oi < no) ? w[8 * oi + jj] : 0.0;
+  }
+  const int64_t base = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * RPW;
+#pragma unroll 2
+  for (int rr = 0; rr < RPW / 4; ++rr) {
+    const int64_t i = base + 4 * rr + rs;
+    const uint4* row = reinterpret_cast<const uint4*>(Q + i * nr);
+    halfx8 q[OT];
+#pragma unroll
+    for (int t = 0; t < OT; ++t) {
+      const int oi = ol + 16 * t;
+      q[t] = __builtin_bit_cast(halfx8, (i < n && oi < no) ? row[oi] : uint4{0u, 0u, 0u, 0u});
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int t = 0; t < OT; ++t)
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) s = fma((double)q[t][jj], wr[t][jj], s);
+    s += __shfl_xor(s, 1, 64);
+    s += __shfl_xor(s, 2, 64);
+    s += __shfl_xor(s, 4, 64);
+    s += __shfl_xor(s, 8, 64);
+    if (ol == 0 && i < n) z[i] = (float)(f * (double)x[i] + s);
+  }
+}
+
+__global__ void to_half_kernel(const float* __restrict__ a, _Float16* __restrict__ out, int64_t count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) out[i] = (_Float16)a[i];
+}
+
 // z[i] = f * x[i] + sum_c Q[i][c] w[c]   (one wave per QW_ROWS rows)
 constexpr int QW_ROWS = 2;
 template <class T>
@@ -762,6 +874,86 @@ int precond_apply(Precond& pc, const void* X, int k, int64_t ldx, void* Z, int64
 static int read_scalar(const void* dev, void* host, size_t bytes, hipStream_t st) {
   LAKER_HIP_CHECK(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, st));
   LAKER_HIP_CHECK(hipStreamSynchronize(st));
+  return LAKER_OK;
+}
+
+// FP16 copy of the CCCP basis for the k = 1 applies of the mixed solve (half
+// the bytes of the two passes over Q).  Rounding Q alone could leave
+// a^{-1/2}(I - Qh Qh^T) slightly indefinite, so the preconditioner is formed
+// on the orthonormal basis Q0 = Qh L^{-T} of span(Qh) instead (L L^T =
+// Qh^T Qh): P' = a^{-1/2}(I - Q0 Q0^T) + Q0 M^{-1/2} Q0^T = a^{-1/2} I +
+// Qh S' Qh^T with S' = L^{-T} S L^{-1}, symmetric positive definite like P and
+// within O(2^-11) of it.  Returns 1 when the handle does not qualify (not an
+// FP32 CCCP handle, N_r not a multiple of 8 or above 640, Qh^T Qh not
+// positive definite), LAKER_OK once Qh and S' exist.
+int precond_prepare_half(Precond& pc, hipStream_t st) {
+  if (pc.has_h) return LAKER_OK;
+  if (pc.kind != PRECOND_CCCP || pc.dtype != LAKER_F32 || pc.nr <= 0 || pc.nr % 8 != 0 || pc.nr > 640) return 1;
+  const int64_t n = pc.n;
+  const int nr = pc.nr;
+  const size_t nn = (size_t)nr * nr;
+  LAKER_TRY(pc.Qh.ensure((size_t)n * nr * sizeof(_Float16)));
+  hipLaunchKernelGGL(to_half_kernel, dim3((unsigned)ceil_div(n * nr, 256)), dim3(256), 0, st, pc.Q.as<float>(),
+                     pc.Qh.as<_Float16>(), n * nr);
+  LAKER_KCHECK("to_half_kernel");
+  DevBuf G, Li, T, info;
+  TriWork tw;
+  LAKER_TRY(G.ensure(nn * 8));
+  LAKER_TRY(Li.ensure(nn * 8));
+  LAKER_TRY(T.ensure(nn * 8));
+  LAKER_TRY(info.ensure(sizeof(int)));
+  LAKER_TRY(pc.Sh.ensure(nn * 8));
+  // Qh^T Qh from the FP32 Q rounded on load (the values of Qh, FP32-wide loads)
+  const F32AsF16* qr = reinterpret_cast<const F32AsF16*>(pc.Q.as<float>());
+  LAKER_TRY((gemm<F32AsF16, F32AsF16, double>(true, false, nr, nr, n, 1.0, qr, nr, qr, nr, 0.0, G.as<double>(), nr,
+                                               &pc.ws, st)));
+  LAKER_HIP_CHECK(hipMemsetAsync(info.ptr, 0, sizeof(int), st));
+  LAKER_TRY(potrf_lower(G.as<double>(), nr, nr, info.as<int>(), &pc.ws, tw, st));
+  LAKER_TRY(set_identity(Li.as<double>(), nr, 1.0, st));
+  LAKER_TRY(trsm_lower_left(G.as<double>(), nr, nr, Li.as<double>(), nr, nr, &pc.ws, tw, st));  // L^{-1}
+  LAKER_TRY((gemm<double, double, double>(false, false, nr, nr, nr, 1.0, pc.S.as<double>(), nr, Li.as<double>(),
+                                           nr, 0.0, T.as<double>(), nr, &pc.ws, st)));  // S L^{-1}
+  LAKER_TRY((gemm<double, double, double>(true, false, nr, nr, nr, 1.0, Li.as<double>(), nr, T.as<double>(), nr,
+                                           0.0, pc.Sh.as<double>(), nr, &pc.ws, st)));  // L^{-T} S L^{-1}
+  int hinfo = 0;
+  LAKER_HIP_CHECK(hipMemcpyAsync(&hinfo, info.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
+  LAKER_HIP_CHECK(hipStreamSynchronize(st));  // before the temporaries are freed
+  if (hinfo != 0) return 1;
+  pc.has_h = true;
+  return LAKER_OK;
+}
+
+// z = P' x for one FP32 vector with the FP16 basis (see precond_prepare_half):
+// the k = 1 path of apply_t with Qh and S'.
+int precond_apply_half(Precond& pc, const float* X, float* Z, hipStream_t st) {
+  if (!pc.has_h) return fail(LAKER_ESTATE, "precond_apply_half: no FP16 basis");
+  const int64_t n = pc.n;
+  const int nr = pc.nr;
+  LAKER_TRY(pc.w2.ensure((size_t)nr * 8));
+  double* W2 = pc.w2.as<double>();
+  const int64_t nb = std::min<int64_t>(1024, ceil_div(n, 64));
+  const int64_t rpb = ceil_div(n, nb);
+  const int nb_eff = (int)ceil_div(n, rpb);
+  constexpr int RG = 8;  // first-level reduction groups
+  LAKER_TRY(pc.part.ensure(((size_t)nb_eff + RG) * nr * 8));
+  double* part = pc.part.as<double>();
+  double* part2 = part + (size_t)nb_eff * nr;
+  const _Float16* qh = pc.Qh.as<_Float16>();
+  const dim3 qg((unsigned)nb_eff), qb(256);
+  if ((nr / 8 + 63) / 64 == 1) hipLaunchKernelGGL((qtx8h_kernel<1>), qg, qb, 0, st, qh, X, n, nr, rpb, part);
+  else hipLaunchKernelGGL((qtx8h_kernel<2>), qg, qb, 0, st, qh, X, n, nr, rpb, part);
+  LAKER_KCHECK("qtx8h_kernel");
+  hipLaunchKernelGGL(reduce_parts_groups_kernel, dim3((unsigned)ceil_div(nr, 64), RG), dim3(256), 0, st,
+                     (const double*)part, nb_eff, nr, part2);
+  LAKER_KCHECK("reduce_parts_groups_kernel");
+  static_assert(RG == 8, "gemv64_sum8_kernel sums eight group partials");
+  hipLaunchKernelGGL(gemv64_sum8_kernel, dim3((unsigned)ceil_div(nr, 8)), dim3(256), nr * sizeof(double), st,
+                     pc.Sh.as<double>(), (const double*)part2, W2, nr);
+  LAKER_KCHECK("gemv64_sum8_kernel");
+  // 128 rows per block (4 waves x 32 rows)
+  hipLaunchKernelGGL((qw8h_kernel<32>), dim3((unsigned)ceil_div(n, 128)), dim3(256), 0, st, qh, (const double*)W2,
+                     X, Z, n, nr, pc.inv_sqrt_a);
+  LAKER_KCHECK("qw8h_kernel");
   return LAKER_OK;
 }
 

@@ -24,6 +24,8 @@ int64_t precond_rows(const struct Precond& pc);
 int precond_dtype(const struct Precond& pc);
 int precond_apply(struct Precond& pc, const void* X, int k, int64_t ldx, void* Z, int64_t ldz,
                   hipStream_t st);
+int precond_prepare_half(struct Precond& pc, hipStream_t st);
+int precond_apply_half(struct Precond& pc, const float* X, float* Z, hipStream_t st);
 
 enum { PCG_RUNNING = 0, PCG_CONVERGED = 1, PCG_BREAKDOWN = 2, PCG_MAXITER = 3 };
 
@@ -809,6 +811,19 @@ int laker_solve_mixed(laker_operator_t oh, laker_precond_t ph, const double* B, 
   LAKER_TRY(pcg_buffers<float>(op, k, b));
   const int nblk = b.nblk;
   DevBuf tmp;
+  // k = 1 with a CCCP handle: the applies read the FP16 basis (half the bytes;
+  // an SPD preconditioner within O(2^-11) of the FP32 one, used throughout
+  // this solve)
+  bool half_apply = false;
+  if (pc && k == 1) {
+    const int hr = precond_prepare_half(*pc, st);
+    if (hr < 0) return hr;
+    half_apply = hr == LAKER_OK;
+  }
+  auto papply = [&](const float* r, float* z) -> int {
+    if (half_apply) return precond_apply_half(*pc, r, z, st);
+    return precond_apply(*pc, r, k, k, z, k, st);
+  };
   const dim3 eg((unsigned)ceil_div(nk, 256)), eb(256);
   const bool clamp = op.clamp_for(LAKER_CLAMP_AUTO);
   const double eps = std::sqrt(1.1920928955078125e-07);
@@ -865,7 +880,7 @@ int laker_solve_mixed(laker_operator_t oh, laker_precond_t ph, const double* B, 
       hipLaunchKernelGGL(pcg_resid_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.st);
       LAKER_KCHECK("pcg_resid_kernel");
     }
-    if (pc) LAKER_TRY(precond_apply(*pc, b.r, k, k, b.z, k, st));
+    if (pc) LAKER_TRY(papply(b.r, b.z));
     else hipLaunchKernelGGL((copy_kernel<float>), eg, eb, 0, st, (const float*)b.r, b.z, nk);
     if (k == 1) {
       // the residual test of this iteration rides on the r.z dot kernel
@@ -895,7 +910,7 @@ int laker_solve_mixed(laker_operator_t oh, laker_precond_t ph, const double* B, 
     const double seg_tol = std::min(0.5, final_step ? need : std::max(inner_tol, need));
     hipLaunchKernelGGL(scale_to_f32_kernel, eg, eb, 0, st, Rin, b.r, nk, 1.0 / rnorm);
     LAKER_KCHECK("scale_to_f32_kernel");
-    if (pc) LAKER_TRY(precond_apply(*pc, b.r, k, k, b.z, k, st));
+    if (pc) LAKER_TRY(papply(b.r, b.z));
     else hipLaunchKernelGGL((copy_kernel<float>), eg, eb, 0, st, (const float*)b.r, b.z, nk);
     hipLaunchKernelGGL((copy_kernel<float>), eg, eb, 0, st, (const float*)b.z, b.p, nk);
     LAKER_KCHECK("copy_kernel");
