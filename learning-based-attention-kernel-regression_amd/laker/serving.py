@@ -70,7 +70,9 @@ def footprint_bytes(n: int, k: int, d: int, nr: Optional[int] = None) -> int:
     ka32, ka16, ka16h = 2 * ((d + 1) // 2), 16 * ((6 * d + 15) // 16), 16 * ((3 * d + 15) // 16)
     npad = 64 * ((n + 63) // 64)
     packs = npad * (2 * ka32 * 4 + 2 * ka16 * 2 + 2 * ka16h * 2) + n * d * 12
-    build = n * nr * (4 + 4 + 8 + 8) + nr * nr * 8 * 12
+    # Q, probe responses and FP64 Cholesky-QR work (n x nr each), the FP16 copy
+    # of Q of the mixed k = 1 applies, and the nr x nr factors
+    build = n * nr * (4 + 4 + 8 + 8 + 2) + nr * nr * 8 * 14
     solve = n * k * (6 * 4 + 3 * 8) + 64 * n * max(k, 4) * 4
     # partial sums of the symmetric k=1 product (column partials ~n^2/512
     # doubles, row partials per 1024-column segment) and the packed BF16
