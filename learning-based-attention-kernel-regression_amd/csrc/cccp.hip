@@ -700,6 +700,95 @@ __global__ __launch_bounds__(256) void qw8h_kernel(const _Float16* __restrict__ 
   }
 }
 
+// G = Qh^T Qh (nr x nr, FP64) for the FP16 basis, on MFMA.  Workgroup
+// (tile pair ti <= tj of 64 columns, row split s) accumulates the 64 x 64 block
+// over its rows: 32-row panels of both column groups are loaded with 16-byte
+// loads, stored transposed in LDS (column-major, pitch GH_P halves) so a lane
+// reads 8 consecutive rows of one column at once, and fed to
+// v_mfma_f32_32x32x16_f16 (products of FP16 values are exact in FP32); the
+// FP32 tile is added into FP64 every 256 rows.  part[s][i][j] (i, j < nr)
+// holds the split's sums; gram_reduce_kernel adds the splits in order.
+constexpr int GH_P = 40;  // LDS pitch (halves) of one transposed column: 32 rows + 8
+__global__ __launch_bounds__(256) void gram_f16_kernel(const _Float16* __restrict__ Q, int64_t n, int nr,
+                                                       int64_t rows_per_split, const int2* __restrict__ pairs,
+                                                       double* __restrict__ part) {
+  __shared__ __align__(16) _Float16 ta[64 * GH_P];
+  __shared__ __align__(16) _Float16 tb[64 * GH_P];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wi = w >> 1, wj = w & 1, h = lane >> 5;
+  const int2 tp = pairs[blockIdx.x];
+  const int c0 = 64 * tp.x, c1 = 64 * tp.y;
+  const int64_t s = blockIdx.y;
+  const int64_t r0 = s * rows_per_split;
+  const int64_t r1 = (r0 + rows_per_split < n) ? r0 + rows_per_split : n;
+  // panel load: thread t takes row t / 8 of the 32, columns 8 (t % 8) .. + 7
+  const int prow = tid >> 3, poct = tid & 7;
+  auto load = [&](int64_t rb, int c, uint4& v) {
+    const int64_t r = rb + prow;
+    const int col = c + 8 * poct;
+    v = (r < r1 && col < nr) ? *reinterpret_cast<const uint4*>(Q + r * nr + col) : uint4{0u, 0u, 0u, 0u};
+  };
+  auto stash = [&](_Float16* t, const uint4& v) {
+    const halfx8 hv = __builtin_bit_cast(halfx8, v);
+#pragma unroll
+    for (int jj = 0; jj < 8; ++jj) t[(8 * poct + jj) * GH_P + prow] = hv[jj];
+  };
+  floatx16 acc = {0};
+  double acc64[16];
+#pragma unroll
+  for (int v = 0; v < 16; ++v) acc64[v] = 0.0;
+  uint4 pa, pb;
+  if (r0 < r1) {
+    load(r0, c0, pa);
+    load(r0, c1, pb);
+  }
+  int steps = 0;
+  for (int64_t rb = r0; rb < r1; rb += 32) {
+    __syncthreads();  // the previous panel's MFMAs are done with the LDS
+    stash(ta, pa);
+    stash(tb, pb);
+    __syncthreads();
+    if (rb + 32 < r1) {
+      load(rb + 32, c0, pa);
+      load(rb + 32, c1, pb);
+    }
+#pragma unroll
+    for (int ss = 0; ss < 2; ++ss) {
+      const halfx8 av = *reinterpret_cast<const halfx8*>(ta + (32 * wi + (lane & 31)) * GH_P + 16 * ss + 8 * h);
+      const halfx8 bv = *reinterpret_cast<const halfx8*>(tb + (32 * wj + (lane & 31)) * GH_P + 16 * ss + 8 * h);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, acc, 0, 0, 0);
+    }
+    if (++steps == 8) {  // 256 rows in FP32, then into FP64
+      steps = 0;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        acc64[v] += (double)acc[v];
+        acc[v] = 0.f;
+      }
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 16; ++v) acc64[v] += (double)acc[v];
+  // D layout: lane l holds column n = l % 32 and rows m = 8 (v / 4) + 4 h + v % 4
+  const int j = c1 + 32 * wj + (lane & 31);
+#pragma unroll
+  for (int v = 0; v < 16; ++v) {
+    const int i = c0 + 32 * wi + 8 * (v >> 2) + 4 * h + (v & 3);
+    if (i < nr && j < nr) part[(s * nr + i) * nr + j] = acc64[v];
+  }
+}
+
+// G[i][j] = sum over splits of part[s][min(i,j)][max(i,j)] (fixed order; the
+// computed blocks are those with ti <= tj, which hold every (min, max) pair)
+__global__ void gram_reduce_kernel(const double* __restrict__ part, int nsplit, int nr, double* __restrict__ G) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)nr * nr) return;
+  const int i = (int)(idx / nr), j = (int)(idx % nr);
+  const int a = i < j ? i : j, b = i < j ? j : i;
+  double v = 0.0;
+  for (int s = 0; s < nsplit; ++s) v += part[((int64_t)s * nr + a) * nr + b];
+  G[idx] = v;
+}
+
 __global__ void to_half_kernel(const float* __restrict__ a, _Float16* __restrict__ out, int64_t count) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) out[i] = (_Float16)a[i];
@@ -883,7 +972,9 @@ static int read_scalar(const void* dev, void* host, size_t bytes, hipStream_t st
 // on the orthonormal basis Q0 = Qh L^{-T} of span(Qh) instead (L L^T =
 // Qh^T Qh): P' = a^{-1/2}(I - Q0 Q0^T) + Q0 M^{-1/2} Q0^T = a^{-1/2} I +
 // Qh S' Qh^T with S' = L^{-T} S L^{-1}, symmetric positive definite like P and
-// within O(2^-11) of it.  Returns 1 when the handle does not qualify (not an
+// within O(2^-11) of it.  (The Gram sums FP16 products exactly in FP32 over
+// 256-row blocks and in FP64 beyond: Q0 is orthonormal to ~1e-6, far inside
+// the margin a^{-1/2}(I - Q0 Q0^T) needs against M^{-1/2}.)  Returns 1 when the handle does not qualify (not an
 // FP32 CCCP handle, N_r not a multiple of 8 or above 640, Qh^T Qh not
 // positive definite), LAKER_OK once Qh and S' exist.
 int precond_prepare_half(Precond& pc, hipStream_t st) {
@@ -903,10 +994,27 @@ int precond_prepare_half(Precond& pc, hipStream_t st) {
   LAKER_TRY(T.ensure(nn * 8));
   LAKER_TRY(info.ensure(sizeof(int)));
   LAKER_TRY(pc.Sh.ensure(nn * 8));
-  // Qh^T Qh from the FP32 Q rounded on load (the values of Qh, FP32-wide loads)
-  const F32AsF16* qr = reinterpret_cast<const F32AsF16*>(pc.Q.as<float>());
-  LAKER_TRY((gemm<F32AsF16, F32AsF16, double>(true, false, nr, nr, n, 1.0, qr, nr, qr, nr, 0.0, G.as<double>(), nr,
-                                               &pc.ws, st)));
+  // Qh^T Qh on FP16 MFMA (exact products, FP32 over 256 rows, FP64 beyond)
+  {
+    const int nt = (int)ceil_div(nr, 64);
+    std::vector<int2> pairs;
+    for (int a = 0; a < nt; ++a)
+      for (int b = a; b < nt; ++b) pairs.push_back(int2{a, b});
+    const int64_t nsplit = std::max<int64_t>(1, std::min<int64_t>(64, ceil_div(n, 2048)));
+    const int64_t rps = round_up(ceil_div(n, nsplit), 32);
+    const int64_t ns_eff = ceil_div(n, rps);
+    DevBuf pairs_d, gpart;
+    LAKER_TRY(pairs_d.ensure(pairs.size() * sizeof(int2)));
+    LAKER_TRY(gpart.ensure((size_t)ns_eff * nn * 8));
+    LAKER_HIP_CHECK(hipMemcpyAsync(pairs_d.ptr, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(gram_f16_kernel, dim3((unsigned)pairs.size(), (unsigned)ns_eff), dim3(256), 0, st,
+                       pc.Qh.as<_Float16>(), n, nr, rps, pairs_d.as<int2>(), gpart.as<double>());
+    LAKER_KCHECK("gram_f16_kernel");
+    hipLaunchKernelGGL(gram_reduce_kernel, dim3((unsigned)ceil_div((int64_t)nn, 256)), dim3(256), 0, st,
+                       (const double*)gpart.as<double>(), (int)ns_eff, nr, G.as<double>());
+    LAKER_KCHECK("gram_reduce_kernel");
+    LAKER_HIP_CHECK(hipStreamSynchronize(st));  // pairs (host vector) and gpart are freed on scope exit
+  }
   LAKER_HIP_CHECK(hipMemsetAsync(info.ptr, 0, sizeof(int), st));
   LAKER_TRY(potrf_lower(G.as<double>(), nr, nr, info.as<int>(), &pc.ws, tw, st));
   LAKER_TRY(set_identity(Li.as<double>(), nr, 1.0, st));
