@@ -6,13 +6,6 @@ namespace laker {
 
 // C = alpha * op(A) * op(B) + beta * C, row-major, FP64 accumulation.
 // op(A) is M x K, op(B) is K x N.  ws (optional) enables split-K.
-// FP32 storage read as its FP16 rounding: the GEMM sees exactly the values of
-// an FP16 copy of the array without the copy being read (4-byte loads).
-struct F32AsF16 {
-  float v;
-  __host__ __device__ explicit operator double() const { return (double)(_Float16)v; }
-};
-
 template <class TA, class TB, class TC>
 int gemm(bool tra, bool trb, int64_t M, int64_t N, int64_t K, double alpha, const TA* A,
          int64_t lda, const TB* B, int64_t ldb, double beta, TC* C, int64_t ldc, DevBuf* ws,

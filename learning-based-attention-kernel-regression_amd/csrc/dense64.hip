@@ -174,7 +174,6 @@ int gemm(bool tra, bool trb, int64_t M, int64_t N, int64_t K, double alpha, cons
                                 int64_t, const TB*, int64_t, double, TC*, int64_t, DevBuf*,  \
                                 hipStream_t);
 LAKER_INST_GEMM(double, double, double)
-LAKER_INST_GEMM(F32AsF16, F32AsF16, double)
 LAKER_INST_GEMM(float, float, double)
 LAKER_INST_GEMM(float, double, double)
 LAKER_INST_GEMM(float, double, float)
