@@ -374,8 +374,11 @@ __global__ __launch_bounds__(256) void reduce_parts_groups_kernel(const double* 
   const int gsz = (nb + gridDim.y - 1) / gridDim.y;
   const int b0 = blockIdx.y * gsz, b1 = min(nb, b0 + gsz);
   double s = 0.0;
-  if (c < nr)
+  if (c < nr) {
+    // same additions in the same order; the unroll only keeps loads in flight
+#pragma unroll 8
     for (int b = b0 + cy; b < b1; b += 4) s += part[(int64_t)b * nr + c];
+  }
   sm[cy][cx] = s;
   __syncthreads();
   if (cy == 0 && c < nr) out[(int64_t)blockIdx.y * nr + c] = (sm[0][cx] + sm[1][cx]) + (sm[2][cx] + sm[3][cx]);
@@ -574,7 +577,8 @@ __global__ __launch_bounds__(256) void gemv64_kernel(const double* __restrict__ 
 // y[r] = sum_c S[r][c] w1[c] with w1 = the sum of the 8 group partials of
 // Q^T x, formed per block in LDS in the order of reduce_parts_groups_kernel
 // ((g0 + g4) + (g1 + g5)) + ((g2 + g6) + (g3 + g7)), so the result matches
-// the separate reduction bit for bit.  Two rows per wave.
+// the separate reduction bit for bit.  GS8_RPW rows per wave.
+constexpr int GS8_RPW = 1;  // rows per wave (the row's sum does not depend on it)
 __global__ __launch_bounds__(256) void gemv64_sum8_kernel(const double* __restrict__ S,
                                                           const double* __restrict__ part8,
                                                           double* __restrict__ y, int nr) {
@@ -586,10 +590,11 @@ __global__ __launch_bounds__(256) void gemv64_sum8_kernel(const double* __restri
   __syncthreads();
   const int lane = threadIdx.x & 63;
 #pragma unroll
-  for (int rr = 0; rr < 2; ++rr) {
-    const int r = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 2 + rr;
+  for (int rr = 0; rr < GS8_RPW; ++rr) {
+    const int r = (blockIdx.x * 4 + (threadIdx.x >> 6)) * GS8_RPW + rr;
     if (r < nr) {  // uniform per wave
       double v = 0.0;
+#pragma unroll 8  // loads in flight; the FMAs keep their order
       for (int c = lane; c < nr; c += 64) v = fma(S[(int64_t)r * nr + c], w1s[c], v);
       v = wave_sum(v);
       if (lane == 0) y[r] = v;
@@ -863,7 +868,7 @@ static int apply_t(Precond& pc, const T* X, int k, int64_t ldx, T* Z, int64_t ld
                          nb_eff, nr, part2);
       LAKER_KCHECK("reduce_parts_groups_kernel");
       static_assert(RG == 8, "gemv64_sum8_kernel sums eight group partials");
-      hipLaunchKernelGGL(gemv64_sum8_kernel, dim3((unsigned)ceil_div(nr, 8)), dim3(256), nr * sizeof(double), st,
+      hipLaunchKernelGGL(gemv64_sum8_kernel, dim3((unsigned)ceil_div(nr, 4 * GS8_RPW)), dim3(256), nr * sizeof(double), st,
                          pc.S.as<double>(), (const double*)part2, W2, nr);
       LAKER_KCHECK("gemv64_sum8_kernel");
       hipLaunchKernelGGL((qw4_kernel<2>), dim3((unsigned)ceil_div(n, 8)), dim3(256), nr * sizeof(double), st,
@@ -1055,7 +1060,7 @@ int precond_apply_half(Precond& pc, const float* X, float* Z, hipStream_t st) {
                      (const double*)part, nb_eff, nr, part2);
   LAKER_KCHECK("reduce_parts_groups_kernel");
   static_assert(RG == 8, "gemv64_sum8_kernel sums eight group partials");
-  hipLaunchKernelGGL(gemv64_sum8_kernel, dim3((unsigned)ceil_div(nr, 8)), dim3(256), nr * sizeof(double), st,
+  hipLaunchKernelGGL(gemv64_sum8_kernel, dim3((unsigned)ceil_div(nr, 4 * GS8_RPW)), dim3(256), nr * sizeof(double), st,
                      pc.Sh.as<double>(), (const double*)part2, W2, nr);
   LAKER_KCHECK("gemv64_sum8_kernel");
   // 128 rows per block (4 waves x 32 rows)

@@ -109,6 +109,7 @@ __global__ __launch_bounds__(256) void coldots_kernel(const T* __restrict__ A0, 
 // lane ends with the same bits).
 __device__ inline double wave_sum_part(const double* part, int q, int nblk, int k, int c, int lane) {
   double s = 0.0;
+#pragma unroll 8  // loads in flight; the additions keep their order
   for (int b = lane; b < nblk; b += 64) s += part[((int64_t)q * nblk + b) * k + c];
   return wave_sum(s);
 }
