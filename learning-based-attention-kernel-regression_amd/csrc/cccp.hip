@@ -52,15 +52,18 @@ __device__ inline uint64_t splitmix64(uint64_t x) {
   x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
   return x ^ (x >> 31);
 }
+// bf16_exact: round each draw to the nearest BF16 value (the probe product of
+// the mixed build then needs no lo part of the probes).
 template <class T>
-__global__ void randn_kernel(T* __restrict__ out, int64_t count, uint64_t seed) {
+__global__ void randn_kernel(T* __restrict__ out, int64_t count, uint64_t seed, int bf16_exact = 0) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   const uint64_t a = splitmix64(seed * 0x632BE59BD9B4E019ull + 2 * (uint64_t)i);
   const uint64_t b = splitmix64(seed * 0x632BE59BD9B4E019ull + 2 * (uint64_t)i + 1);
   const double u1 = ((a >> 11) + 1.0) * (1.0 / 9007199254740993.0);
   const double u2 = (b >> 11) * (1.0 / 9007199254740992.0);
-  out[i] = (T)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+  const T v = (T)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+  out[i] = bf16_exact ? (T)bf16_bits_to_f32(f32_to_bf16_bits((float)v)) : v;
 }
 
 // column sums of squares of an n x m row-major matrix -> out[m] (FP64), grid = m
@@ -1347,11 +1350,14 @@ int laker_precond_build(laker_operator_t oh, int32_t nr, double gamma, double ep
   // 1. probes and responses U = A R
   DevBuf probe_buf, U;
   const void* Rp = probes;
+  // probes drawn here for the hi/lo probe product are BF16-exact Gaussians,
+  // so that product needs two BF16 products instead of three
+  const bool xbf16 = !Rp && op.dtype == LAKER_F32 && probe_precision == LAKER_PREC_HILO;
   if (!Rp) {
     LAKER_TRY(probe_buf.ensure((size_t)n * nr * esz));
     if (op.dtype == LAKER_F32)
       hipLaunchKernelGGL((randn_kernel<float>), dim3((unsigned)ceil_div(n * nr, 256)), dim3(256), 0, st,
-                         probe_buf.as<float>(), n * nr, seed);
+                         probe_buf.as<float>(), n * nr, seed, xbf16 ? 1 : 0);
     else
       hipLaunchKernelGGL((randn_kernel<double>), dim3((unsigned)ceil_div(n * nr, 256)), dim3(256), 0, st,
                          probe_buf.as<double>(), n * nr, seed);
@@ -1363,6 +1369,7 @@ int laker_precond_build(laker_operator_t oh, int32_t nr, double gamma, double ep
   if (op.dtype == LAKER_F32) {
     int prec = probe_precision;
     if (prec == LAKER_PREC_FP64) prec = LAKER_PREC_FP32;
+    if (xbf16) prec = PREC_HILO_XBF16;
     LAKER_TRY(fused_product_f32(op, prec, op.rows(), op.cols(), (const float*)Rp, nr, nr, U.as<float>(),
                                 nr, op.lambda_reg, (const float*)Rp, nr, clamp, false, 0, st));
   } else {

@@ -87,6 +87,11 @@ int pack_store(SideStore& s, const void* E, int dtype, int64_t n, int d, int ka3
 // dots3 (optional, k = 1): when the square operator's symmetric kernel runs,
 // its reduction also writes the per-256-row-block partials of Xr.Y, Xr.Xr and
 // Y.Y to dots3[q * ceil(n / 256) + block] and sets *dots3_done.
+// Internal product precision (not part of the C ABI): the exp tile split hi/lo
+// times an X whose entries are exactly representable in BF16 (two BF16
+// products, hi.X + lo.X; the lo part of X would be zero).
+constexpr int PREC_HILO_XBF16 = 7;
+
 int fused_product_f32(Operator& op, int precision, const Side& rows, const Side& cols,
                       const float* X, int k, int64_t ldx, float* Y, int64_t ldy, double lam,
                       const float* Xr, int64_t ldxr, bool clamp, bool deriv, int accumulate,

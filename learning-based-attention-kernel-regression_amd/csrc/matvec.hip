@@ -983,6 +983,7 @@ __global__ void pack_x_kernel(const float* __restrict__ X, int64_t ldx, int64_t 
 }
 
 constexpr int GX_SPLIT3 = 5;  // exp tile and X both split 3-way, 6 BF16 products (~2^-24)
+constexpr int GX_PLXH = 6;    // exp tile split hi/lo, X exactly BF16: 2 products (hi.X + lo.X)
 
 template <int GX>
 struct M2Parts {
@@ -1105,7 +1106,7 @@ __global__ __launch_bounds__(256) void mv_multi2_kernel(
               const float pv = kexp<CLAMP, false>(D[it][8 * u + v], clampv);
               const uint16_t hb = f32_to_bf16_bits(pv);
               Ph[u][v] = __builtin_bit_cast(__bf16, hb);
-              if constexpr (GX == LAKER_PREC_MIXED)
+              if constexpr (GX == LAKER_PREC_MIXED || GX == GX_PLXH)
                 Pl[u][v] = __builtin_bit_cast(__bf16, f32_to_bf16_bits(pv - bf16_bits_to_f32(hb)));
               if constexpr (GX == GX_SPLIT3) {
                 const float r1 = pv - bf16_bits_to_f32(hb);
@@ -1126,6 +1127,8 @@ __global__ __launch_bounds__(256) void mv_multi2_kernel(
                 acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Pm[u], Xh[q][u], acc[it][q], 0, 0, 0);
                 acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ph[u], Xm[q][u], acc[it][q], 0, 0, 0);
               }
+              if constexpr (GX == GX_PLXH)
+                acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Pl[u], Xh[q][u], acc[it][q], 0, 0, 0);
               acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ph[u], Xh[q][u], acc[it][q], 0, 0, 0);
               if constexpr (GX == LAKER_PREC_MIXED) {
                 acc[it][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Pl[u], Xh[q][u], acc[it][q], 0, 0, 0);
@@ -1831,7 +1834,8 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
   if (dots3_done) *dots3_done = false;
   LAKER_CHECK_ARG(k >= 1, "k must be >= 1");
   const float clampv = (float)(op.clamp_limit * kLog2eD);
-  const bool mixed = precision == LAKER_PREC_MIXED || precision == LAKER_PREC_BF16 || precision == LAKER_PREC_HILO;
+  const bool mixed = precision == LAKER_PREC_MIXED || precision == LAKER_PREC_BF16 || precision == LAKER_PREC_HILO ||
+                     precision == PREC_HILO_XBF16;
   const void* colpack = mixed ? cols.pack16 : cols.pack32;
   const void* rowpack = mixed ? rows.pack16 : rows.pack32;
   const int ka = mixed ? op.ka16 : op.ka32;
@@ -1888,6 +1892,7 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
   const int gx = precision == LAKER_PREC_FP32 ? LAKER_PREC_FP32
                  : precision == LAKER_PREC_BF16 ? LAKER_PREC_BF16
                  : precision == LAKER_PREC_HILO ? LAKER_PREC_MIXED
+                 : precision == PREC_HILO_XBF16 ? GX_PLXH
                                                 : GX_SPLIT3;
   // columns per launch: up to 64 (NCC=2) per pass; passes recompute the Gram tile.
   const int64_t kp = round_up(k, 32);
@@ -1916,7 +1921,7 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
       const int rem = k - c0;
       const int nq = (gx != GX_SPLIT3 && rem > 64) ? 4 : (rem > 32 ? 2 : 1);
       hipLaunchKernelGGL(pack_x_kernel, dim3((unsigned)ceil_div(ntiles * 128 * nq, 256)), dim3(256), 0, st, X,
-                         ldx, n_cols, k, c0, ntiles, nq, xh, gx == LAKER_PREC_BF16 ? nullptr : xl,
+                         ldx, n_cols, k, c0, ntiles, nq, xh, (gx == LAKER_PREC_BF16 || gx == GX_PLXH) ? nullptr : xl,
                          gx == GX_SPLIT3 ? xm : nullptr);
       LAKER_KCHECK("pack_x_kernel");
 #define LAKER_M2J(NS, GXM, NC, JTV)                                                                   \
@@ -1979,6 +1984,7 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
       const int gns = f16s ? ns16h : ns16;
       if (gx == GX_SPLIT3) LAKER_M2_NS3(GX_SPLIT3);
       else if (gx == LAKER_PREC_MIXED) LAKER_M2_NS(LAKER_PREC_MIXED);
+      else if (gx == GX_PLXH) LAKER_M2_NS(GX_PLXH);
       else LAKER_M2_NS(LAKER_PREC_BF16);
 #undef LAKER_M2_NS
 #undef LAKER_M2_NS3
