@@ -230,19 +230,28 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double* __restrict__ A,
   }
   for (int i = rg; i < nb; i += 4)
     if (c < nb) A[(k0 + i) * lda + k0 + c] = (c <= i) ? L[i][c] : 0.0;
-  // column t of L^{-1} by forward substitution (entries above t stay zero)
-  if (tid < PNB) {
-    const int t = tid;
-    double x[PNB];
+  // column t of L^{-1} by forward substitution (entries above t stay zero),
+  // column-oriented: once x_p is known, every later row i takes
+  // v_i -= L[i][p] x_p, so each v_i receives its terms in the order p = 0, 1, ...
+  // of the row-oriented dot product while the rows update in parallel.  Four
+  // threads (adjacent lanes of one wave) share column t = tid / 4; thread r
+  // keeps the rows i = 4m + r, and x_p comes from the lane that owns row p.
+  {
+    const int t = tid >> 2, r = tid & 3, lane = tid & 63;
+    double v[PNB / 4];
 #pragma unroll
-    for (int i = 0; i < PNB; ++i) {
-      double v = (i == t) ? 1.0 : 0.0;
+    for (int m = 0; m < PNB / 4; ++m) v[m] = (4 * m + r == t) ? 1.0 : 0.0;
 #pragma unroll
-      for (int p = 0; p < i; ++p) v -= L[i][p] * x[p];
-      x[i] = (i < t) ? 0.0 : v / L[i][i];
+    for (int p = 0; p < PNB; ++p) {
+      const int m0 = p >> 2;
+      double xo = v[m0] / L[p][p];
+      xo = __shfl(xo, (lane & ~3) | (p & 3), 64);
+      const double xp = (p < t) ? 0.0 : xo;
+      if (r == (p & 3)) dinv[p * PNB + t] = xp;
+#pragma unroll
+      for (int m = m0; m < PNB / 4; ++m)
+        if (4 * m + r > p) v[m] -= L[4 * m + r][p] * xp;
     }
-#pragma unroll
-    for (int i = 0; i < PNB; ++i) dinv[i * PNB + t] = x[i];
   }
 }
 
