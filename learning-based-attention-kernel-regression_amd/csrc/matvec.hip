@@ -1917,9 +1917,11 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
     const dim3 block(256);
     for (int c0 = 0; c0 < k;) {
       // column chunks of 32 per pass: the 3-way split keeps 2 for its register budget;
-      // hi/lo and BF16 take 4 when more than 64 columns remain (half the Gram passes)
+      // hi/lo and BF16 take 4 when more than 64 columns remain (half the Gram passes);
+      // BF16-exact X with the hi/lo exp tile (two products) takes 8 when more than
+      // 128 remain (248 VGPRs, one column tile per LDS chunk)
       const int rem = k - c0;
-      const int nq = (gx != GX_SPLIT3 && rem > 64) ? 4 : (rem > 32 ? 2 : 1);
+      const int nq = (gx == GX_PLXH && rem > 128) ? 8 : (gx != GX_SPLIT3 && rem > 64) ? 4 : (rem > 32 ? 2 : 1);
       hipLaunchKernelGGL(pack_x_kernel, dim3((unsigned)ceil_div(ntiles * 128 * nq, 256)), dim3(256), 0, st, X,
                          ldx, n_cols, k, c0, ntiles, nq, xh, (gx == LAKER_PREC_BF16 || gx == GX_PLXH) ? nullptr : xl,
                          gx == GX_SPLIT3 ? xm : nullptr);
@@ -1928,7 +1930,7 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
   do {                                                                                                \
     constexpr size_t lds2 = 2 * (size_t)m2_chunk_bytes(NS, M2Parts<GXM>::value, NC, JTV);            \
     /* one row tile per wave for the register-heavy 3-way split and 4-chunk passes */                \
-    constexpr int ITWV = (GXM == GX_SPLIT3 || NC == 4) ? 1 : 2;                                       \
+    constexpr int ITWV = (GXM == GX_SPLIT3 || NC >= 4) ? 1 : 2;                                       \
     dim3 grid2((unsigned)(ceil_div(n_rows, 128 * ITWV) * jsplit2));                                   \
     if (f16s) {                                                                                       \
       if (clamp)                                                                                      \
@@ -1966,6 +1968,18 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
     if (nq == 2) LAKER_M2C(NS, GXM, 2);                \
     else LAKER_M2C(NS, GXM, 1);                        \
   } while (0)
+#define LAKER_M2_8(NS, GXM)                            \
+  do {                                                 \
+    if (nq == 8) LAKER_M2C(NS, GXM, 8);                \
+    else LAKER_M2(NS, GXM);                            \
+  } while (0)
+#define LAKER_M2_NS8(GXM)                    \
+  do {                                       \
+    if (gns == 2) LAKER_M2_8(2, GXM);        \
+    else if (gns == 3) LAKER_M2_8(3, GXM);   \
+    else if (gns == 4) LAKER_M2_8(4, GXM);   \
+    else LAKER_M2_8(6, GXM);                 \
+  } while (0)
 #define LAKER_M2_NS(GXM)                     \
   do {                                       \
     if (gns == 2) LAKER_M2(2, GXM);          \
@@ -1984,9 +1998,11 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
       const int gns = f16s ? ns16h : ns16;
       if (gx == GX_SPLIT3) LAKER_M2_NS3(GX_SPLIT3);
       else if (gx == LAKER_PREC_MIXED) LAKER_M2_NS(LAKER_PREC_MIXED);
-      else if (gx == GX_PLXH) LAKER_M2_NS(GX_PLXH);
+      else if (gx == GX_PLXH) LAKER_M2_NS8(GX_PLXH);
       else LAKER_M2_NS(LAKER_PREC_BF16);
 #undef LAKER_M2_NS
+#undef LAKER_M2_NS8
+#undef LAKER_M2_8
 #undef LAKER_M2_NS3
 #undef LAKER_M2
 #undef LAKER_M2_2
