@@ -56,6 +56,54 @@ def read_traffic(n, d):
     return vals["hbm_bytes_per_launch"]
 
 
+def cpu_c4_step_time(O, laker, n_tr, n_val, steps=3, lam=1.0):
+    """Median wall time of one C4 training step on the host, restated with the
+    oracle: embed, CCCP + fp32 PCG solve, validation MSE of the prediction, its
+    gradient through the cross kernel (autograd) and the implicit term
+    -v^T (dA/dtheta) alpha with v from one adjoint PCG solve."""
+    import torch
+
+    x, y = laker.pathloss_problem(n_tr + n_val, seed=0)
+    x_tr, y_tr, x_va, y_va = x[:n_tr], y[:n_tr], x[n_tr:], y[n_tr:]
+    torch.manual_seed(0)
+    net = torch.nn.Sequential(torch.nn.Linear(2, 64), torch.nn.Tanh(), torch.nn.Linear(64, 64), torch.nn.Tanh(),
+                              torch.nn.Linear(64, 16))
+    for mod in net:
+        if isinstance(mod, torch.nn.Linear):
+            torch.nn.init.xavier_uniform_(mod.weight)
+            torch.nn.init.zeros_(mod.bias)
+    opt = torch.optim.Adam(net.parameters(), lr=1e-3)
+    nr = O.default_num_probes(n_tr)
+    times = []
+    for _ in range(steps + 1):
+        t0 = time.perf_counter()
+        opt.zero_grad()
+        e = net(x_tr / 100.0)
+        eq = net(x_va / 100.0)
+        ed = e.detach()
+
+        def op(v):
+            return O.attention_matvec(ed, lam, v)
+
+        fac = O.cccp_build(op, n_tr, O.reference_probes(n_tr, nr, None), 0.1, 1e-8, 0.05, 200, 1e-6)
+
+        def pre(v):
+            return O.cccp_apply(v, fac)
+
+        alpha = O.pcg_solve(op, pre, y_tr, tol=1e-6, max_iter=1000).x
+        pred = torch.exp(eq @ e.T) @ alpha
+        r = pred - y_va
+        loss = torch.dot(r, r) / n_val
+        with torch.no_grad():
+            g_alpha = torch.exp(ed @ eq.detach().T) @ ((2.0 / n_val) * r.detach())
+            v = O.pcg_solve(op, pre, g_alpha, tol=1e-6, max_iter=1000).x
+        implicit = -torch.dot(v, torch.exp(e @ e.T) @ alpha + lam * alpha)
+        (loss + implicit).backward()
+        opt.step()
+        times.append(time.perf_counter() - t0)
+    return statistics.median(times[1:])
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,13 +122,13 @@ def parse():
     ap.add_argument("--no-oracle", action="store_true")
     ap.add_argument("--matvec-only", type=int, default=0,
                     help="only launch the k=1 matvec this many times per precision (profiling)")
-    ap.add_argument("--serving-requests", type=int, default=24,
-                    help="requests in the bounded serving sample (0 disables it)")
+    ap.add_argument("--serving-requests", type=int, default=200,
+                    help="requests in the serving sample: ids 0..R-1 of the stream (0 disables it)")
     ap.add_argument("--serving-slots", type=int, default=4)
-    ap.add_argument("--serving-rate", type=float, default=0.0,
+    ap.add_argument("--serving-rate", type=float, default=4.0,
                     help="Poisson arrival rate (requests/s, seed 0) of the serving sample, honoured against "
                          "the wall clock; 0 makes every request available at t=0 (saturating)")
-    ap.add_argument("--serving-seq", type=int, default=0,
+    ap.add_argument("--serving-seq", type=int, default=40,
                     help="requests of the sequential fp32 comparison (0: the same count as the sample)")
     ap.add_argument("--quick", action="store_true")
     return ap.parse_args()
@@ -278,6 +326,29 @@ def main():
         preo.close()
         torch.cuda.empty_cache()
 
+    # C0 on the GPU: the full fit + predict + score of the n=2000 configuration
+    # (reference defaults, fp32), median of 5 after one warm-up
+    x0, y0 = laker.pathloss_problem(2000, seed=0)
+    xt0 = torch.rand(64, 2, generator=torch.Generator().manual_seed(1)) * 100.0
+
+    def c0_gpu():
+        m = laker.LAKERRegressor(embedding_dim=10, lambda_reg=1e-2, gamma=0.1, pcg_tol=1e-6, pcg_max_iter=1000,
+                                 verbose=False)
+        m.fit(x0, y0)
+        m.predict(xt0)
+        m.score(x0, y0)
+        torch.cuda.synchronize()
+        return m
+
+    c0_gpu()
+    t_c0 = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        m0 = c0_gpu()
+        t_c0.append(time.perf_counter() - t0)
+    c0 = {"workload": "fit + predict(64) + score at n=2000 d=10 lambda=1e-2 fp32 (reference defaults)",
+          "gpu_s_median": statistics.median(t_c0), "gpu_pcg_iterations": m0.pcg_iterations_}
+
     # CPU baseline: the oracle (torch-CPU restatement of the reference path) on host cores
     cpu = None
     if not args.no_cpu:
@@ -286,29 +357,6 @@ def main():
 
         threads = len(os.sched_getaffinity(0))
         torch.set_num_threads(threads)
-        e_cpu = e.float().cpu()
-        y_cpu = y.float()
-        rows = min(n, 4096)
-        chunk = O.auto_chunk(n)
-        t0 = time.perf_counter()
-        # one 1-D matvec row block (the reference's 2-D tile loop over all columns)
-        acc = torch.zeros(rows)
-        for j0 in range(0, n, chunk or n):
-            j1 = min(j0 + (chunk or n), n)
-            g = O.exp_safe(e_cpu[:rows] @ e_cpu[j0:j1].T, True)
-            acc += g @ y_cpu[j0:j1]
-        t_rows = time.perf_counter() - t0
-        t_mv_cpu = t_rows * n / rows
-        probe_rows = min(n, 256)
-        pr = torch.randn(n, nr)
-        t0 = time.perf_counter()
-        for j0 in range(0, n, chunk or n):
-            j1 = min(j0 + (chunk or n), n)
-            g = O.exp_safe(e_cpu[:probe_rows] @ e_cpu[j0:j1].T, True)
-            _ = g @ pr[j0:j1]
-        t_probe_cpu = (time.perf_counter() - t0) * n / probe_rows
-        iters = res.get("fp32", {}).get("iterations", inf_m[-1]["iterations"])
-        t_solve_cpu = t_probe_cpu + iters * t_mv_cpu
         cpu_model = "unknown"
         try:
             for line in open("/proc/cpuinfo"):
@@ -317,12 +365,78 @@ def main():
                     break
         except OSError:
             pass
+        # C0: full fit + predict + score, median of 5 after one warm-up (not extrapolated)
+        def c0_cpu():
+            r = O.fit(x0, y0)
+            q = r["embedding_model"](xt0)
+            O.predict(q, r["embeddings"], r["alpha"], r["skip_clamp"])
+            qt = r["embedding_model"](x0)
+            pred_train = O.predict(qt, r["embeddings"], r["alpha"], r["skip_clamp"])
+            float(-torch.sqrt(torch.mean((pred_train - y0) ** 2)))  # score = -RMSE
+            return r
+
+        c0_cpu()
+        t_c0c = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            r0 = c0_cpu()
+            t_c0c.append(time.perf_counter() - t0)
+        c0.update({"cpu_s_median": statistics.median(t_c0c), "cpu_pcg_iterations": r0["iterations"],
+                   "cpu_kind": "port (oracle fit/predict/score)"})
+        print(f"[bench] C0 fit+predict+score: GPU {c0['gpu_s_median'] * 1e3:.0f} ms, CPU "
+              f"{c0['cpu_s_median'] * 1e3:.0f} ms ({threads} threads)", file=sys.stderr, flush=True)
+
+        # C1: one full k=1 product and one full N_r-column probe product with the
+        # oracle's own tiling (the reference's 2-D tiles, chunk 8192 at n=100,000)
+        e_cpu = e.float().cpu()
+        y_cpu = y.float()
+        chunk = O.auto_chunk(n)
+        skip = O.skip_clamp_flag(e_cpu)
+        t0 = time.perf_counter()
+        O.attention_matvec(e_cpu, args.lambda_reg, y_cpu, chunk, skip)
+        t_mv_cpu = time.perf_counter() - t0
+        pr = torch.randn(n, nr, generator=torch.Generator().manual_seed(0))
+        t0 = time.perf_counter()
+        O.attention_matvec(e_cpu, args.lambda_reg, pr, chunk, skip)
+        t_probe_cpu = time.perf_counter() - t0
+        del pr
+        iters = res.get("fp32", {}).get("iterations", inf_m[-1]["iterations"])
+        t_solve_cpu = t_probe_cpu + iters * t_mv_cpu
+        print(f"[bench] CPU C1: matvec {t_mv_cpu:.2f} s, probe product {t_probe_cpu:.2f} s", file=sys.stderr,
+              flush=True)
+
+        # C2 per iteration: one 512-column product at n=50,000, d=16 (2-D tiles, chunk 5000)
+        n2, m2 = 50000, 512
+        x2l, _ = laker.pathloss_problem(n2, seed=0)
+        emb16 = laker.PositionEmbedding(2, 16, device=dev)
+        with torch.no_grad():
+            e2 = emb16(x2l.to(dev)).float().cpu()
+        X2 = torch.randn(n2, m2, generator=torch.Generator().manual_seed(1))
+        t0 = time.perf_counter()
+        O.attention_matvec(e2, 1e-3, X2, O.auto_chunk(n2), O.skip_clamp_flag(e2))
+        t_c2_iter = time.perf_counter() - t0
+        del X2, e2
+        print(f"[bench] CPU C2: one 512-column product {t_c2_iter:.2f} s", file=sys.stderr, flush=True)
+
+        # C4 per step at reduced size: 2,000 train + 500 validation points, the
+        # 2->64->64->16 tanh MLP, fp32 PCG with CCCP (lambda 1.0), direct term by
+        # autograd through the cross kernel, implicit term by one adjoint solve
+        t_c4 = cpu_c4_step_time(O, laker, 2000, 500, steps=3)
+        print(f"[bench] CPU C4 step (2000+500): {t_c4:.2f} s", file=sys.stderr, flush=True)
+
         cpu = {"value": 1.0 / t_solve_cpu, "unit": "solves/sec", "cores": torch.get_num_threads(),
                "kind": "port",
-               "sample": (f"oracle fp32 matvec on {rows} of {n} rows (all columns, reference tiling) and the "
-                          f"{nr}-RHS probe matvec on {probe_rows} rows, extrapolated to one solve = probe + "
-                          f"{iters} PCG matvecs; {cpu_model}"),
-               "matvec_s": t_mv_cpu, "probe_matvec_s": t_probe_cpu}
+               "sample": (f"oracle (the reference's tiling, fp32) on {cpu_model}: one full k=1 product at "
+                          f"n={n} ({t_mv_cpu:.1f} s) and one full {nr}-column probe product ({t_probe_cpu:.1f} s), "
+                          f"extrapolated to one solve = probe + {iters} PCG products"),
+               "matvec_s": t_mv_cpu, "probe_matvec_s": t_probe_cpu, "extrapolated": True,
+               "c0_fit_predict_score_s": c0["cpu_s_median"],
+               "c2_iteration_s": t_c2_iter,
+               "c2_note": "one 512-column product at n=50,000, d=16 (the batched PCG iteration's dominant cost); "
+                          "a solve takes about 380 such iterations (extrapolation)",
+               "c4_step_s_reduced": t_c4,
+               "c4_note": "one training step at 2,000 train + 500 validation points; the products scale as n^2, "
+                          f"so 20,000 + 5,000 points take about {100 * t_c4:.0f} s per step (extrapolation)"}
 
     # Bounded sample of the serving stream: requests r = 0..R-1 (n log-uniform in
     # [1000, 65536], d in {10, 16}, 1..32 RHS, all seeded by r), all available at
@@ -381,6 +495,7 @@ def main():
         "fp64_oracle": oracle,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "c0": c0,
         "serving": serving,
         "host": platform.node(),
     }

@@ -92,7 +92,11 @@ int laker_residual_fp64(laker_operator_t op, const double* B, const double* X, i
 /* Y = exp(min(scale * Eq E^T, clamp)) X for m query rows (no lambda term).
  * Replaces the predict path kernel_eval(q, E) @ alpha (laker/core.py:580-661,
  * laker/kernels.py:301-354) without materialising m x n.
- * clamp_policy LAKER_CLAMP_AUTO uses the operator's skip-clamp flag. */
+ * clamp_policy LAKER_CLAMP_AUTO uses the operator's skip-clamp flag.
+ * Eq has the operator's dtype.  X and Y have the operator's dtype, except
+ * with precision LAKER_PREC_FP64 on an LAKER_F32 operator: then X and Y are
+ * FP64 and the product runs on the FP64 copy of E in FP64 (predictions from
+ * the FP64 solution of laker_solve_mixed), with the FP32 clamp limit. */
 int laker_cross_matvec(laker_operator_t op, const void* Eq, int64_t m, const void* X, int32_t k,
                        int64_t ldx, void* Y, int64_t ldy, int32_t clamp_policy, int32_t precision,
                        hipStream_t stream);
@@ -196,10 +200,15 @@ int laker_precond_from_factors(const void* Q, int64_t n, int32_t nr, int32_t dty
 int laker_orthonormalize(void* X, int64_t n, int32_t k, int32_t dtype, hipStream_t stream);
 
 /* count N(0,1) draws (LAKER_F32/F64) into device memory: the same stream
- * laker_precond_build draws when it is given no probe matrix, so a caller
- * can transform those probes first (the "power_iter" strategy) and pass them
- * back.  Replaces the device torch.randn of laker/preconditioner.py:168-171
- * for probe_source="device" (a different generator than torch's). */
+ * laker_precond_build draws when it is given no probe matrix and its
+ * probe_precision is LAKER_PREC_FP32, _MIXED or _BF16, so a caller can
+ * transform those probes first (the "power_iter" strategy) and pass them
+ * back.  With LAKER_PREC_HILO on an FP32 operator the build rounds each of
+ * these draws to the nearest BF16 value before use (its probe product then
+ * takes two BF16 products instead of three); probes passed in by the caller
+ * are used as given.  Replaces the device torch.randn of
+ * laker/preconditioner.py:168-171 for probe_source="device" (a different
+ * generator than torch's). */
 int laker_randn(void* out, int64_t count, int32_t dtype, uint64_t seed, hipStream_t stream);
 
 /* ---------------------------------------------------------------------------

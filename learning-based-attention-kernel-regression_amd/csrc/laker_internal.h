@@ -38,7 +38,12 @@ struct Operator {
   SideStore self;
   SideStore query;  // packed query set of the last cross product
   std::mutex query_mu;
-  DevBuf part;      // split-K partial sums
+  DevBuf part;      // split-K partial sums of the FP32 / split-precision products
+  // Partial sums of the FP64 products (refinement residual, FP64 operators).
+  // Kept apart from `part` so that an FP64 residual between replays of a
+  // captured graph of FP32 products never reallocates a buffer the graph
+  // still points to.
+  DevBuf part64;
   DevBuf scratch;   // backward right-hand sides
   DevBuf pcg_ws;    // PCG vectors, partial sums and device scalars
   DevBuf mixed_ws;  // refinement residual and FP32 correction vectors

@@ -1906,13 +1906,19 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
   const bool packed_x =
       mixed && !deriv && gx != LAKER_PREC_FP32 && (f16s || ns16 == 2 || ns16 == 4 || ns16 == 6);
   if (packed_x) {
-    // X parts packed per pass: 3 x 2 chunks (3-way split) or 2 x 4 chunks (hi/lo, BF16)
+    // X parts packed per pass, each region sized for the widest pass its
+    // mode takes (pack_x_kernel writes ntiles * 128 * nq vectors per part):
+    // 3 x 2 chunks (3-way split), 2 x 4 chunks (hi/lo), 1 x 4 (BF16) or
+    // 1 x 8 (BF16-exact X)
     const int64_t ntiles = ceil_div(n_cols, 32);
-    const size_t part_bytes = (size_t)ntiles * (gx == GX_SPLIT3 ? 256 : 512) * sizeof(bf16x8);
-    LAKER_TRY(op.xpack.ensure(3 * part_bytes));
+    const int nq_max = gx == GX_SPLIT3 ? 2 : (gx == GX_PLXH ? 8 : 4);
+    const size_t part_bytes = (size_t)ntiles * 128 * nq_max * sizeof(bf16x8);
+    // parts written: hi only (BF16, BF16-exact X), hi + lo (hi/lo), hi + lo + mid (3-way)
+    const int nparts = gx == GX_SPLIT3 ? 3 : (gx == LAKER_PREC_MIXED ? 2 : 1);
+    LAKER_TRY(op.xpack.ensure(nparts * part_bytes));
     bf16x8* xh = op.xpack.as<bf16x8>();
-    bf16x8* xl = reinterpret_cast<bf16x8*>(op.xpack.as<char>() + part_bytes);
-    bf16x8* xm = reinterpret_cast<bf16x8*>(op.xpack.as<char>() + 2 * part_bytes);
+    bf16x8* xl = nparts >= 2 ? reinterpret_cast<bf16x8*>(op.xpack.as<char>() + part_bytes) : nullptr;
+    bf16x8* xm = nparts >= 3 ? reinterpret_cast<bf16x8*>(op.xpack.as<char>() + 2 * part_bytes) : nullptr;
     const int jsplit2 = jsplit;
     const dim3 block(256);
     for (int c0 = 0; c0 < k;) {
@@ -1923,8 +1929,7 @@ int fused_product_f32(Operator& op, int precision, const Side& rows, const Side&
       const int rem = k - c0;
       const int nq = (gx == GX_PLXH && rem > 128) ? 8 : (gx != GX_SPLIT3 && rem > 64) ? 4 : (rem > 32 ? 2 : 1);
       hipLaunchKernelGGL(pack_x_kernel, dim3((unsigned)ceil_div(ntiles * 128 * nq, 256)), dim3(256), 0, st, X,
-                         ldx, n_cols, k, c0, ntiles, nq, xh, (gx == LAKER_PREC_BF16 || gx == GX_PLXH) ? nullptr : xl,
-                         gx == GX_SPLIT3 ? xm : nullptr);
+                         ldx, n_cols, k, c0, ntiles, nq, xh, xl, xm);
       LAKER_KCHECK("pack_x_kernel");
 #define LAKER_M2J(NS, GXM, NC, JTV)                                                                   \
   do {                                                                                                \
@@ -2050,9 +2055,9 @@ int fused_product_f64(Operator& op, const Side& rows, const Side& cols, const XT
     const int64_t n = n_rows;
     const int64_t nseg = ceil_div(ceil_div(n, 16), S64_SEG);
     const int64_t ncb = ceil_div(n, S64_RB) - 1;
-    LAKER_TRY(op.ensure_part((size_t)nseg * n * sizeof(double)));
+    LAKER_TRY(op.part64.ensure((size_t)nseg * n * sizeof(double)));
     LAKER_TRY(op.part_c64.ensure((size_t)std::max<int64_t>(1, s64_coff(ncb, n)) * sizeof(double)));
-    double* part_r = op.part.as<double>();
+    double* part_r = op.part64.as<double>();
     double* part_c = op.part_c64.as<double>();
     const int2* map = op.sym64_map.as<int2>();
     dim3 grid((unsigned)op.sym64_nwg), block(256);
@@ -2076,8 +2081,8 @@ int fused_product_f64(Operator& op, const Side& rows, const Side& cols, const XT
     LAKER_KCHECK("mv_sym64_reduce_kernel");
     return LAKER_OK;
   }
-  LAKER_TRY(op.ensure_part((size_t)jsplit * n_rows * k * sizeof(double)));
-  double* part = op.part.as<double>();
+  LAKER_TRY(op.part64.ensure((size_t)jsplit * n_rows * k * sizeof(double)));
+  double* part = op.part64.as<double>();
   if (op.d <= 16 && (clamp || tmax < 1e9)) {
     const int steps = (op.d + 3) / 4;
     const double rscale = op.scale * kLog2eD * 256.0;
@@ -2354,7 +2359,9 @@ int laker_cross_matvec(laker_operator_t h, const void* Eq, int64_t m, const void
   Side qrows = Operator::view(op->query, true);
   Side cols = op->cols();
   if (op->dtype == LAKER_F64 || precision == LAKER_PREC_FP64) {
-    LAKER_CHECK_ARG(op->dtype == LAKER_F64, "FP64 cross product needs an FP64 operator");
+    // FP64 operators, or an FP32 operator's FP64 copy of E with FP64 X and Y
+    // (predictions from an FP64-refined solution; the clamp limit stays the
+    // operator's)
     return fused_product_f64<double>(*op, qrows, cols, (const double*)X, k, ldx, (double*)Y,
                                      ldy, 0.0, nullptr, 0, clamp, nullptr, stream);
   }

@@ -18,6 +18,7 @@ import torch.nn as nn
 
 from . import _native as N
 from .backend import compute_device, get_default_device, get_default_dtype
+from .distributed_kernels import DistributedAttentionKernelOperator
 from .embeddings import PositionEmbedding
 from .kernels import AttentionKernelOperator, KernelOperator
 from .preconditioner import AdaptivePreconditioner, CCCPPreconditioner
@@ -31,6 +32,14 @@ def auto_chunk(n: int, chunk_size: Optional[int]) -> Optional[int]:
     if chunk_size is None and n > 5000:
         return max(1024, min(n // 10, 8192))
     return chunk_size
+
+
+def native_operator(op):
+    """The HIP-backed operator behind ``op`` (the inner operator of the
+    single-device distributed wrapper), or ``op`` itself."""
+    if isinstance(op, DistributedAttentionKernelOperator):
+        return op.local_op
+    return op
 
 
 class _CrossFn(torch.autograd.Function):
@@ -152,11 +161,19 @@ class LAKERCore:
             raise ValueError(
                 f"kernel_approx={self.kernel_approx!r} is not provided by the HIP solver core; "
                 "pass kernel_approx=None (exact kernel)")
-        if self.distributed:
-            raise ValueError("distributed=True is not provided (one GPU per process)")
         lam = float(lambda_reg) if lambda_reg is not None else self.lambda_reg
         chunk = auto_chunk(embeddings.shape[0], chunk_size)
+        if self.verbose and chunk_size is None and chunk is not None:
+            logger.info("Auto-selected chunk_size=%d for n=%d", chunk, embeddings.shape[0])
         inner = "mixed" if self.precision == "mixed" else "fp32"
+        if self.distributed:
+            # one GPU per process: the wrapper takes the reference's single-device
+            # form (laker/distributed_kernels.py:116-134, laker/core.py:348-360)
+            op = DistributedAttentionKernelOperator(embeddings, lambda_reg=lam, master_device=self.device,
+                                                    dtype=self.dtype, precision=inner)
+            if self.verbose:
+                logger.info("Using distributed kernel on %d device(s)", len(op.devices))
+            return op
         return AttentionKernelOperator(embeddings, lambda_reg=lam, chunk_size=chunk, device=self.device,
                                        dtype=self.dtype, precision=inner)
 
@@ -190,8 +207,9 @@ class LAKERCore:
                                               precision=self.precision, inner_tol=self.inner_tol)
         alpha = pcg.solve(operator=kernel_operator.matvec, preconditioner=preconditioner.apply, rhs=rhs, x0=x0)
         self.last_solver = pcg
-        if self.verbose:
-            r = kernel_operator.residual_fp64(rhs, pcg.solution_fp64 if pcg.solution_fp64 is not None else alpha)
+        native = native_operator(kernel_operator)
+        if self.verbose and isinstance(native, AttentionKernelOperator):
+            r = native.residual_fp64(rhs, pcg.solution_fp64 if pcg.solution_fp64 is not None else alpha)
             rel = float(torch.linalg.norm(r) / torch.linalg.norm(rhs.to(r.device, torch.float64)))
             logger.info("LAKER fit complete: PCG iters=%d, final rel_res=%.3e", pcg.iterations, rel)
         return alpha, pcg.iterations
@@ -218,8 +236,13 @@ class LAKERCore:
         with torch.no_grad():
             q = self._query_embeddings(x, embedding_model)
             clamp_always = self._predict_clamp_always(q.shape[0], embeddings.shape[0])
-            if isinstance(kernel_operator, AttentionKernelOperator):
-                out = kernel_operator.cross_matvec(q, alpha, clamp_always=clamp_always)
+            native = native_operator(kernel_operator)
+            if isinstance(native, AttentionKernelOperator):
+                # an FP64 solution (the mixed solve's refined alpha) is applied in
+                # FP64 on the operator's FP64 copy of E, then returned in dtype
+                fp64 = alpha.dtype == torch.float64 and native.dtype == torch.float32
+                out = native.cross_matvec(q, alpha, clamp_always=clamp_always,
+                                          precision="fp64" if fp64 else None).to(self.dtype)
             else:
                 out = kernel_operator.kernel_eval(q, embeddings) @ alpha
             if residual_corrector is not None:
@@ -233,7 +256,7 @@ class LAKERCore:
         """Differentiable predict (laker/core.py:782-849) with the hand-written cross backward."""
         q = self._query_embeddings(x, embedding_model)
         clamp_always = self._predict_clamp_always(q.shape[0], embeddings.shape[0])
-        out = cross_kernel_matvec(kernel_operator, q, alpha, clamp_always=clamp_always,
+        out = cross_kernel_matvec(native_operator(kernel_operator), q, alpha, clamp_always=clamp_always,
                                   embeddings=embeddings if embeddings.requires_grad else None)
         if residual_corrector is not None:
             residual_corrector.eval()

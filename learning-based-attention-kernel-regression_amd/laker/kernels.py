@@ -234,11 +234,13 @@ class AttentionKernelOperator:
         """``exp(min(s Eq E^T, c)) x`` without materialising (predict, laker/core.py:580-661)."""
         dev = self.embeddings.device
         q = query_embeddings.to(device=dev, dtype=self.dtype).contiguous()
-        x2, one_d = _as_2d(x.to(device=dev, dtype=self.dtype).contiguous())
         prec = _PRECISIONS[precision or "fp32"]
         if self.dtype == torch.float64:
             prec = N.PREC_FP64
-        y = torch.empty(q.shape[0], x2.shape[1], dtype=self.dtype, device=dev)
+        # FP64 products take FP64 vectors (also on an FP32 operator: its FP64 copy of E)
+        vdt = torch.float64 if prec == N.PREC_FP64 else self.dtype
+        x2, one_d = _as_2d(x.to(device=dev, dtype=vdt).contiguous())
+        y = torch.empty(q.shape[0], x2.shape[1], dtype=vdt, device=dev)
         policy = N.CLAMP_ALWAYS if clamp_always else N.CLAMP_AUTO
         N.check(N.lib.laker_cross_matvec(self._h.h, N.ptr(q), q.shape[0], N.ptr(x2), x2.shape[1],
                                          x2.shape[1], N.ptr(y), y.shape[1], policy, prec, N.stream_ptr()))

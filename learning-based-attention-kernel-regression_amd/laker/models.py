@@ -161,7 +161,9 @@ class LAKERRegressor:
         x = to_tensor(x, device=self.device, dtype=self.dtype)
         if x.dim() != 2:
             raise ValueError(f"x must be 2-D, got shape {x.shape}")
-        return self.core.predict(x, self.embedding_model, self.embeddings, self.kernel_operator, self.alpha,
+        # the mixed solve's FP64 solution, where there is one, keeps its accuracy in the predictions
+        alpha = self.alpha_fp64 if self.alpha_fp64 is not None else self.alpha
+        return self.core.predict(x, self.embedding_model, self.embeddings, self.kernel_operator, alpha,
                                  self.residual_corrector)
 
     def predict_variance(self, x) -> torch.Tensor:

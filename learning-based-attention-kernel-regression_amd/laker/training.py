@@ -83,6 +83,8 @@ class EmbeddingTrainer:
         regressor.kernel_operator = op
         regressor.preconditioner = precond
         regressor.alpha, regressor.pcg_iterations_ = self.core.solve_pcg(op, precond, yd)
+        solver = getattr(self.core, "last_solver", None)
+        regressor.alpha_fp64 = solver.solution_fp64 if solver is not None else None
         return regressor
 
     def validation_step(self, model, x_tr, y_tr, x_val, y_val, precond=None, rebuild: bool = True):

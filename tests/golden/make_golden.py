@@ -150,8 +150,76 @@ def main(ref_root):
     es = torch.tensor([[0.051, 0.452]], dtype=torch.float64)
     np.savez(os.path.join(HERE, "paper_n3.npz"), e=e3.numpy(), y=y3.numpy(), e_star=es.numpy(),
              lambda_reg=np.array(0.1))
+    well_conditioned(ref_root)
+    reference_model_file(ref_root)
     print("golden vectors written to", HERE)
 
 
+def well_conditioned(ref_root):
+    """6. The reference's fp32 fit on a well-conditioned instance.
+
+    n=2000 path-loss problem (seed 0) with lambda_reg=100 and pcg_tol=1e-7
+    (kappa(lambda I + K) = 24 in FP64): the reference's own fp32 solution is
+    within 2e-7 of the FP64 solution on alpha and 4e-6 on predictions, so a
+    1e-5 relative agreement with it is a meaningful bound for another fp32
+    solve.  kappa is stored with the vectors.
+    """
+    sys.path.insert(0, ref_root)
+    from laker.models import LAKERRegressor
+
+    torch.set_num_threads(8)
+    x, y = pathloss_problem(2000, seed=0)
+    gen = torch.Generator().manual_seed(1)
+    xt = torch.rand(64, 2, generator=gen) * 100.0
+    model = LAKERRegressor(embedding_dim=10, lambda_reg=100.0, gamma=0.1, pcg_tol=1e-7,
+                           pcg_max_iter=1000, verbose=False, device="cpu", dtype=torch.float32)
+    model.fit(x, y)
+    pred = model.predict(xt)
+    e = model.embeddings.double().numpy()
+    a = np.exp(e @ e.T)
+    a[np.diag_indices_from(a)] += 100.0
+    w = np.linalg.eigvalsh(a)
+    np.savez(
+        os.path.join(HERE, "fit_n2000_lam100.npz"),
+        x=x.numpy(), y=y.numpy(), xt=xt.numpy(), lambda_reg=np.array(100.0), pcg_tol=np.array(1e-7),
+        embeddings=model.embeddings.numpy(), alpha=model.alpha.numpy(),
+        iterations=np.array(model.pcg_iterations_), pred=pred.numpy(),
+        kappa=np.array(w[-1] / w[0]),
+    )
+
+
+def reference_model_file(ref_root):
+    """7. A model file written by the reference's ModelPersistence.save.
+
+    n=500 fp32 fit (seed 0, reference defaults) saved with torch.save by the
+    reference's own code (laker/persistence.py:74-137), plus its predictions
+    at 64 query points, so a loader can be checked against a file it did not
+    write.  The file holds tensors, strings, numbers and the embedding
+    module's state dict only, so torch.load(weights_only=True) reads it.
+    """
+    sys.path.insert(0, ref_root)
+    from laker.models import LAKERRegressor
+    from laker.persistence import ModelPersistence
+
+    torch.set_num_threads(8)
+    x, y = pathloss_problem(500, seed=0)
+    gen = torch.Generator().manual_seed(2)
+    xt = torch.rand(64, 2, generator=gen) * 100.0
+    model = LAKERRegressor(embedding_dim=10, lambda_reg=1e-2, gamma=0.1, pcg_tol=1e-6,
+                           pcg_max_iter=1000, verbose=False, device="cpu", dtype=torch.float32)
+    model.fit(x, y)
+    path = os.path.join(HERE, "reference_model_n500.pt")
+    ModelPersistence.save(model, path)
+    np.savez(os.path.join(HERE, "reference_model_n500_pred.npz"), xt=xt.numpy(),
+             pred=model.predict(xt).numpy(), alpha=model.alpha.numpy(), embeddings=model.embeddings.numpy())
+
+
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    only = sys.argv[2] if len(sys.argv) > 2 else None
+    if only == "well_conditioned":
+        well_conditioned(root)
+    elif only == "reference_model":
+        reference_model_file(root)
+    else:
+        main(root)

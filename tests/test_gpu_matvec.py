@@ -215,8 +215,10 @@ def test_full_size_rows_subset(cuda, n):
     g = np.exp(e64[rows] @ e64.T)
     ref = 0.01 * yv[rows] + g @ yv
     s = 0.01 * np.abs(yv[rows]) + g @ np.abs(yv)
-    for precision, tol in (("fp32", 2e-5), ("mixed", 3e-4)):
+    for precision, tol in (("fp32", 2e-5), ("mixed", 2e-5)):
         op = laker.AttentionKernelOperator(e, lambda_reg=0.01, precision=precision)
         out = op.matvec(y.to(cuda)).double().cpu().numpy()[rows]
-        assert float(np.max(np.abs(out - ref) / s)) < tol, precision
+        err = float(np.max(np.abs(out - ref) / s))
+        print(f"n={n} {precision}: max row error {err:.2e}", flush=True)
+        assert err < tol, precision
         op.close()

@@ -31,23 +31,14 @@ def fp64_variance(e, q, lam):
 def test_variance_c2_fp32_and_mixed(cuda):
     x, y = laker.pathloss_problem(N_TRAIN, seed=0)
     xq = torch.rand(M, 2, generator=torch.Generator().manual_seed(1)) * 100.0
-    out, breakdown = {}, {}
+    out = {}
     for precision in ("mixed", "fp32"):
         t0 = time.perf_counter()
         model = laker.LAKERRegressor(embedding_dim=D, lambda_reg=LAM, pcg_tol=1e-6, pcg_max_iter=1000,
                                      verbose=False, precision=precision, probe_source="device",
                                      probe_precision="mixed" if precision == "mixed" else "fp32")
         model.fit(x, y)
-        try:
-            var = model.predict_variance(xq)
-        except RuntimeError as exc:
-            # the reference's batched PCG raises on non-positive curvature in
-            # any column (laker/solvers.py:334-385); in FP32 at lambda=1e-3 the
-            # operator's rounding error can exceed lambda ||p||^2
-            assert precision == "fp32" and "PCG breakdown" in str(exc), exc
-            breakdown[precision] = str(exc)
-            print(f"variance {precision}: {exc}", flush=True)
-            continue
+        var = model.predict_variance(xq)
         torch.cuda.synchronize()
         solver = model.core.last_solver
         print(f"variance {precision}: {time.perf_counter() - t0:.1f} s, batched PCG iterations "
@@ -66,9 +57,7 @@ def test_variance_c2_fp32_and_mixed(cuda):
               flush=True)
     # At lambda = 1e-3 and n = 50,000 the FP32-level operator error exceeds
     # lambda, so refinement cannot contract to 1e-6: the mixed solve keeps its
-    # best FP64 iterate and must be at least as accurate as plain FP32.
-    assert "mixed" in out and bool((out["mixed"] >= 0).all()) and out["mixed"].shape == (M,)
-    assert errs["mixed"] < 2e-3, errs
-    if "fp32" in errs:
-        assert errs["fp32"] < 5e-2, errs
-        assert errs["mixed"] <= 1.5 * errs["fp32"] + 1e-6, errs
+    # best FP64 iterate.  Both legs are measured at about 1e-6 of k(q,q).
+    assert bool((out["mixed"] >= 0).all()) and out["mixed"].shape == (M,)
+    assert errs["mixed"] < 1e-5, errs
+    assert errs["fp32"] < 1e-5, errs
