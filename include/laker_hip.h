@@ -83,6 +83,21 @@ int laker_operator_set_lambda(laker_operator_t op, double lambda_reg);
 int laker_matvec(laker_operator_t op, const void* X, int32_t k, int64_t ldx, void* Y, int64_t ldy,
                  int32_t precision, hipStream_t stream);
 
+/* Allocation-free Y = (lambda I + G) X for an LAKER_F32 operator, 1 <= k <= 4
+ * and precision LAKER_PREC_FP32 or LAKER_PREC_MIXED: the same result as
+ * laker_matvec to FP32 rounding (split-K row products instead of the
+ * symmetric k = 1 kernel).  No reference counterpart: it is for callers that
+ * cannot allocate device memory during the call (HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  It calls no hipMalloc,
+ * hipFree or allocating routine; its partial sums live in a 16 MiB device
+ * array that the library holds on every GPU it runs on until the process
+ * ends, so n <= 262144 for k <= 2 and n <= 131072 for k = 3, 4 (LAKER_EINVAL
+ * beyond).  All calls on one GPU share that array: calls that overlap on one
+ * GPU (concurrent streams or threads) race and must be serialised by the
+ * caller. */
+int laker_matvec_noalloc(laker_operator_t op, const float* X, int32_t k, int64_t ldx, float* Y, int64_t ldy,
+                         int32_t precision, hipStream_t stream);
+
 /* R = B - (lambda I + G) X with FP64 X/B/R and FP64 Gram/exp/accumulation:
  * the iterative-refinement residual (no reference counterpart; it checks the
  * reference's stopping quantity ||b - A x|| / ||b||, laker/solvers.py:266-267). */

@@ -2202,6 +2202,91 @@ int pack_store(SideStore& s, const void* E, int dtype, int64_t n, int d, int ka3
   return pack_store_t<double>(s, (const double*)E, n, d, ka32, ka16, cscale, need_col, need_row, st);
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free products (laker_matvec_noalloc).  Their split-K partial sums
+// live in `scratch`, a file-scope device array that every GPU the library
+// runs on holds until the process ends.  Host code never passes `scratch`
+// itself (its host-side name is not a device address): the device address
+// comes from hipGetSymbolAddress, once per device with that device current,
+// and is kept in scratch_dev[device].
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;  // per GPU
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex scratch_mu;
+static void* scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (resolved on first use).
+static int scratch_for_current_device(void** out) {
+  int dev = -1;
+  LAKER_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(LAKER_EINVAL, "device ordinal " + std::to_string(dev) + " is beyond the scratch table");
+  std::lock_guard<std::mutex> g(scratch_mu);
+  if (scratch_dev[dev] == nullptr) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr) {
+      (void)hipGetLastError();
+      return fail(LAKER_EHIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    }
+    scratch_dev[dev] = ptr;
+  }
+  *out = scratch_dev[dev];
+  return LAKER_OK;
+}
+
+// Partial-sum bytes the allocation-free product needs for n rows and k <= 4
+// columns: FP64 partials of the LDS-staged k = 1 kernel, FP32 partials with
+// stride 1, 2 or 4 for k = 2..4.
+static size_t noalloc_part_bytes(int64_t n, int k) {
+  const int jsplit = pick_jsplit(n, n, 256);
+  const int kcp = k == 1 ? 1 : (k == 2 ? 2 : 4);
+  return k == 1 ? (size_t)jsplit * n * sizeof(double) : (size_t)jsplit * n * kcp * sizeof(float);
+}
+
+// Y = lambda X + exp(min(s E E^T, c)) X for an FP32 operator and k <= 4,
+// with the partial sums in the caller-provided device buffer `part`
+// (noalloc_part_bytes(n, k) bytes).  Every partial the reduction reads is
+// written by the product kernel first, so `part` needs no clearing.  No
+// device memory is allocated.
+static int product_vec_into(Operator& op, int precision, const float* X, int k, int64_t ldx, float* Y,
+                            int64_t ldy, void* part, hipStream_t st) {
+  const bool clamp = op.clamp_for(LAKER_CLAMP_AUTO);
+  const float clampv = (float)(op.clamp_limit * kLog2eD);
+  const bool mixed = precision == LAKER_PREC_MIXED;
+  const Side rows = op.rows(), cols = op.cols();
+  const void* colpack = mixed ? cols.pack16 : cols.pack32;
+  const void* rowpack = mixed ? rows.pack16 : rows.pack32;
+  const int ka = mixed ? op.ka16 : op.ka32;
+  const int gmode = mixed ? LAKER_PREC_MIXED : LAKER_PREC_FP32;
+  const int64_t n = op.n;
+  const int jsplit = pick_jsplit(n, n, 256);
+  const int64_t tot = n * k;
+  if (k == 1) {
+    const int rc = launch_vec1(gmode, colpack, rowpack, ka, n, n, X, ldx, (double*)part, jsplit, clamp, clampv,
+                               nullptr, st);
+    if (rc < 0) return rc;
+    if (rc == 0) {
+      hipLaunchKernelGGL((mv_reduce_kernel<double, float, float, false>), dim3((unsigned)ceil_div(tot, 256)),
+                         dim3(256), 0, st, (const double*)part, n, 1, (int64_t)1, jsplit, op.lambda_reg, X, ldx,
+                         (const float*)nullptr, Y, ldy, 0, (const int*)nullptr);
+      LAKER_KCHECK("mv_reduce_kernel");
+      return LAKER_OK;
+    }
+    // no LDS-staged instance for this width: FP32 partials (half the bytes)
+  }
+  LAKER_TRY(launch_vec(gmode, colpack, rowpack, ka, n, n, X, ldx, k, (float*)part, jsplit, clamp, false, clampv,
+                       nullptr, st));
+  const int kcp = k == 1 ? 1 : (k == 2 ? 2 : 4);
+  hipLaunchKernelGGL((mv_reduce_kernel<float, float, float, false>), dim3((unsigned)ceil_div(tot, 256)),
+                     dim3(256), 0, st, (const float*)part, n, k, (int64_t)kcp, jsplit, op.lambda_reg, X, ldx,
+                     (const float*)nullptr, Y, ldy, 0, (const int*)nullptr);
+  LAKER_KCHECK("mv_reduce_kernel");
+  return LAKER_OK;
+}
+
 }  // namespace laker
 
 using namespace laker;
@@ -2290,6 +2375,12 @@ int laker_operator_create(const void* E, int64_t n, int32_t d, int32_t dtype, do
   }
   op->max_sq_norm = __builtin_bit_cast(double, bits);
   op->skip_clamp = (op->max_sq_norm * scale) < op->clamp_limit;
+  // set-up of the allocation-free products on this device (once per device)
+  void* scratch_ptr = nullptr;
+  if ((rc = scratch_for_current_device(&scratch_ptr)) != LAKER_OK) {
+    delete op;
+    return rc;
+  }
   // max |e_ij| <= max ||e_i||: both split sides stay below 2^14, far from FP16 overflow
   op->f16s_ok = std::sqrt(op->max_sq_norm) * std::max(1.0, std::fabs(scale * kLog2eD)) < 16384.0;
   *out = reinterpret_cast<laker_operator_t>(op);
@@ -2333,6 +2424,28 @@ int laker_matvec(laker_operator_t h, const void* X, int32_t k, int64_t ldx, void
                   "bad precision");
   return fused_product_f32(*op, precision, rows, cols, (const float*)X, k, ldx, (float*)Y, ldy,
                            op->lambda_reg, (const float*)X, ldx, clamp, false, 0, stream);
+}
+
+int laker_matvec_noalloc(laker_operator_t h, const float* X, int32_t k, int64_t ldx, float* Y, int64_t ldy,
+                         int32_t precision, hipStream_t stream) {
+  LAKER_CHECK_ARG(h != nullptr, "operator handle is NULL");
+  LAKER_CHECK_ARG(X != nullptr && Y != nullptr, "X/Y pointer is NULL");
+  LAKER_CHECK_ARG(k >= 1 && k <= 4, "laker_matvec_noalloc takes 1 <= k <= 4 right-hand sides");
+  LAKER_CHECK_ARG(ldx >= k && ldy >= k, "bad leading dimension");
+  LAKER_CHECK_ARG(precision == LAKER_PREC_FP32 || precision == LAKER_PREC_MIXED,
+                  "laker_matvec_noalloc takes LAKER_PREC_FP32 or LAKER_PREC_MIXED");
+  Operator* op = reinterpret_cast<Operator*>(h);
+  LAKER_CHECK_ARG(op->dtype == LAKER_F32, "laker_matvec_noalloc needs an FP32 operator");
+  const size_t need = noalloc_part_bytes(op->n, k);
+  if (need > kNoAllocScratchBytes)
+    return fail(LAKER_EINVAL, "laker_matvec_noalloc: n = " + std::to_string(op->n) + " with k = " +
+                                  std::to_string(k) + " needs " + std::to_string(need) +
+                                  " bytes of scratch, above its maximum of " +
+                                  std::to_string(kNoAllocScratchBytes) +
+                                  " (n <= 262144 for k <= 2, n <= 131072 for k = 3, 4)");
+  void* part = nullptr;
+  LAKER_TRY(scratch_for_current_device(&part));
+  return product_vec_into(*op, precision, (const float*)X, k, ldx, (float*)Y, ldy, part, stream);
 }
 
 int laker_residual_fp64(laker_operator_t h, const double* B, const double* X, int32_t k, double* R,

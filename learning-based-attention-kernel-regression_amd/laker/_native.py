@@ -82,6 +82,8 @@ def _load():
         "laker_operator_set_lambda": (c_int32, [c_void_p, c_double]),
         "laker_matvec": (c_int32, [c_void_p, c_void_p, c_int32, c_int64, c_void_p, c_int64, c_int32,
                                    c_void_p]),
+        "laker_matvec_noalloc": (c_int32, [c_void_p, c_void_p, c_int32, c_int64, c_void_p, c_int64, c_int32,
+                                           c_void_p]),
         "laker_residual_fp64": (c_int32, [c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p]),
         "laker_cross_matvec": (c_int32, [c_void_p, c_void_p, c_int64, c_void_p, c_int32, c_int64,
                                          c_void_p, c_int64, c_int32, c_int32, c_void_p]),
@@ -137,7 +139,8 @@ lib = _load()
 
 EXPORTED = [
     "laker_last_error", "laker_version", "laker_operator_create", "laker_operator_destroy",
-    "laker_operator_info", "laker_operator_set_lambda", "laker_matvec", "laker_residual_fp64",
+    "laker_operator_info", "laker_operator_set_lambda", "laker_matvec", "laker_matvec_noalloc",
+    "laker_residual_fp64",
     "laker_cross_matvec", "laker_cross_matvec_t", "laker_kernel_eval", "laker_diagonal",
     "laker_to_dense", "laker_matvec_backward", "laker_cross_matvec_backward", "laker_precond_build",
     "laker_precond_build_from_responses", "laker_precond_copy_out", "laker_precond_destroy",
