@@ -14,7 +14,8 @@ ROOT = os.path.dirname(PKG_DIR)
 CSRC = os.path.join(ROOT, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB_NAME = "liblaker_hip.so"
-SOURCES = ["runtime.hip", "matvec.hip", "dense64.hip", "cccp.hip", "pcg.hip", "backward.hip"]
+SOURCES = ["runtime.hip", "matvec.hip", "matvec_sym.hip", "matvec_multi.hip", "matvec_f64.hip", "dense64.hip",
+           "cccp.hip", "pcg.hip", "backward.hip"]
 ARCH = os.environ.get("LAKER_OFFLOAD_ARCH", "gfx950")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 

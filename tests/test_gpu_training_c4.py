@@ -70,3 +70,71 @@ def test_validation_training_full_size(cuda):
     print(f"validation MSE {losses[0]:.4f} -> {losses[-1]:.4f} (min {min(losses):.4f}); "
           f"PCG iterations {iters[0]} .. {iters[-1]}", flush=True)
     assert min(losses[-10:]) < losses[0]
+
+
+def test_lambda_1e2_fp32_breakdown_and_mixed_solves(cuda):
+    """The same training at the n = 2000 configuration's lambda = 1e-2.
+
+    The embedding drifts (max ||e||^2 from 1.1 to above 5 in under ten Adam
+    steps), u_fp32 ||K|| passes lambda and the plain fp32 PCG -- the
+    reference's arithmetic and breakdown test (laker/solvers.py:243-246) --
+    raises its breakdown error (measured: step 9, max ||e||^2 = 6.2).  Run by
+    the mixed solver, the same steps' solves of A alpha = y reach an FP64
+    relative residual <= 1e-6 while max ||e||^2 <= 5; beyond that (step 8,
+    max ||e||^2 = 5.5, u_fp32 ||K|| ~ 6e-2 > lambda) the refinement cannot
+    contract either, and the solver says so (converged False, residual kept
+    at its best FP64 iterate).  Fifty steps at lambda = 1e-2 therefore need
+    an FP64 inner operator; the full-size run above uses lambda = 1.0."""
+    x, y = laker.pathloss_problem(N_TRAIN + N_VAL, seed=0)
+    x_tr, y_tr, x_va, y_va = x[:N_TRAIN], y[:N_TRAIN], x[N_TRAIN:], y[N_TRAIN:]
+    y64 = y_tr.to(cuda, torch.float64)
+    results = {}
+    for precision in ("fp32", "mixed"):
+        torch.manual_seed(0)
+        model = _Mlp().to(device=cuda, dtype=torch.bfloat16)
+        core = laker.core.LAKERCore(lambda_reg=1e-2, gamma=0.1, pcg_tol=1e-6, pcg_max_iter=1000, verbose=False,
+                                    embedding_dtype=torch.bfloat16, precision=precision)
+        trainer = EmbeddingTrainer(core)
+        opt = torch.optim.Adam(model.parameters(), lr=1e-3)
+        steps, broke, norms, relres, conv = 0, None, [], [], []
+        stop = results.get("fp32", {}).get("broke", 15)
+        for step in range(15 if precision == "fp32" else stop):
+            opt.zero_grad()
+            with torch.no_grad():
+                e = model(x_tr.to(cuda, torch.bfloat16)).float()
+                norms.append(float((e * e).sum(1).max()))
+            try:
+                loss, precond, alpha, it = trainer.validation_step(model, x_tr, y_tr, x_va, y_va, None,
+                                                                   rebuild=True)
+            except RuntimeError as exc:
+                assert "PCG breakdown" in str(exc), exc
+                broke = step
+                break
+            if precision == "mixed":
+                # the solve of A alpha = y at this step, checked in FP64
+                with torch.no_grad():
+                    op = core.build_kernel_operator(e.contiguous())
+                    pre = core.build_preconditioner(op.matvec, e.shape[0])
+                    a, _ = core.solve_pcg(op, pre, y_tr.to(cuda))
+                    sol = core.last_solver.solution_fp64
+                    conv.append(core.last_solver.converged)
+                    r = op.residual_fp64(y64, sol)
+                    relres.append(float(torch.linalg.norm(r) / torch.linalg.norm(y64)))
+                    op.close()
+                    pre.close()
+            opt.step()
+            steps += 1
+        results[precision] = {"broke": broke, "steps": steps, "max_sq_norms": norms, "relres": relres,
+                              "converged": conv}
+        print(f"lambda=1e-2 {precision}: {steps} steps, breakdown at step {broke}; max ||e||^2 "
+              f"{[round(v, 2) for v in norms]}; FP64 residuals of the solves {['%.1e' % v for v in relres]}, "
+              f"converged {conv}", flush=True)
+    f = results["fp32"]
+    assert f["broke"] is not None and f["max_sq_norms"][f["broke"]] > 5.0
+    m = results["mixed"]
+    assert m["broke"] is None and m["steps"] == f["broke"]
+    for nrm, rel, ok in zip(m["max_sq_norms"], m["relres"], m["converged"]):
+        # converged means the FP64 target was met; otherwise the solver reports it
+        assert (ok and rel <= 1e-6) or not ok, (nrm, rel, ok)
+        if nrm <= 5.0:
+            assert ok and rel <= 1e-6, (nrm, rel)
