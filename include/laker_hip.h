@@ -278,6 +278,37 @@ int laker_solve_mixed(laker_operator_t op, laker_precond_t pc, const double* B, 
                       int32_t inner_max_iter, int32_t inner_precision, hipStream_t stream,
                       laker_mixed_result* result);
 
+/* ---------------------------------------------------------------------------
+ * Batched mixed-precision solves: a batch runs the inner PCG iterations of
+ * many small systems -- uneven n, 2 <= k <= 32 right-hand sides each, an FP32
+ * operator and an FP32 CCCP handle each -- as shared kernel launches (one
+ * launch per kernel of the iteration for all systems whose refinement step is
+ * running), and takes each system's refinement steps (FP64 residual, best
+ * iterate, next step) on the host between polls, with the arithmetic and
+ * stopping rules of laker_solve_mixed.  A system's iterates do not depend on
+ * which other systems share the batch.  Systems join with laker_batch_add
+ * (their set-up runs at once) whenever a slot is free, and leave when
+ * laker_batch_result collects them; the operator, preconditioner, B (n x k
+ * FP64) and X (n x k FP64) must stay valid until then.  All work is ordered
+ * on the batch's stream; one host thread drives a batch.  No reference
+ * counterpart: the serving loop's continuous batching
+ * (the reference solves one system per call, laker/models.py:392-455). */
+typedef struct laker_batch_s* laker_batch_t;
+int laker_batch_create(int32_t max_systems, hipStream_t stream, laker_batch_t* out);
+int laker_batch_destroy(laker_batch_t batch);
+int laker_batch_add(laker_batch_t batch, laker_operator_t op, laker_precond_t pc, const double* B, int32_t k,
+                    double* X, double tol, int32_t max_outer, double inner_tol, int32_t inner_max_iter,
+                    int32_t* slot);
+/* Runs `iterations` inner iterations of every running system, then the
+ * refinement bookkeeping of the systems whose step ended; writes the slots
+ * whose solve has finished (results ready) to finished[0 .. *nfinished) and
+ * the number of systems still solving to *nactive. */
+int laker_batch_step(laker_batch_t batch, int32_t iterations, int32_t* finished, int32_t* nfinished,
+                     int32_t* nactive);
+/* Result of a finished slot; frees the slot.  LAKER_EBREAKDOWN as
+ * laker_solve_mixed returns it. */
+int laker_batch_result(laker_batch_t batch, int32_t slot, laker_mixed_result* result);
+
 /* HIP-graph replay of the device PCG iterations (laker_pcg_solve,
  * laker_solve_mixed): after a first eager batch, each batch of polled
  * iterations is one captured graph launch (iteration state lives on the

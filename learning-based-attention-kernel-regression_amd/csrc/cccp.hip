@@ -17,6 +17,7 @@
 #include "dense64.h"
 #include "laker_common.h"
 #include "laker_internal.h"
+#include "batch.h"
 
 namespace laker {
 
@@ -369,13 +370,13 @@ __global__ __launch_bounds__(256) void qtx4_kernel(const float* __restrict__ Q, 
 
 // out[g][c] = sum_{b in group g} part[b][c] with groups of ceil(nb / gridDim.y)
 // rows; block = 64 columns x 4 row phases, fixed order.
-__global__ __launch_bounds__(256) void reduce_parts_groups_kernel(const double* __restrict__ part, int nb,
-                                                                  int nr, double* __restrict__ out) {
+__device__ __forceinline__ void reduce_parts_groups_kernel_body(const double* __restrict__ part, int nb,
+                                                                  int nr, double* __restrict__ out, int bx_, int by_, int gy_) {
   __shared__ double sm[4][64];
   const int cx = threadIdx.x & 63, cy = threadIdx.x >> 6;
-  const int c = blockIdx.x * 64 + cx;
-  const int gsz = (nb + gridDim.y - 1) / gridDim.y;
-  const int b0 = blockIdx.y * gsz, b1 = min(nb, b0 + gsz);
+  const int c = bx_ * 64 + cx;
+  const int gsz = (nb + gy_ - 1) / gy_;
+  const int b0 = by_ * gsz, b1 = min(nb, b0 + gsz);
   double s = 0.0;
   if (c < nr) {
     // same additions in the same order; the unroll only keeps loads in flight
@@ -384,7 +385,12 @@ __global__ __launch_bounds__(256) void reduce_parts_groups_kernel(const double* 
   }
   sm[cy][cx] = s;
   __syncthreads();
-  if (cy == 0 && c < nr) out[(int64_t)blockIdx.y * nr + c] = (sm[0][cx] + sm[1][cx]) + (sm[2][cx] + sm[3][cx]);
+  if (cy == 0 && c < nr) out[(int64_t)by_ * nr + c] = (sm[0][cx] + sm[1][cx]) + (sm[2][cx] + sm[3][cx]);
+}
+
+__global__ __launch_bounds__(256) void reduce_parts_groups_kernel(const double* __restrict__ part, int nb,
+                                                                  int nr, double* __restrict__ out) {
+  reduce_parts_groups_kernel_body(part, nb, nr, out, (int)blockIdx.x, (int)blockIdx.y, (int)gridDim.y);
 }
 
 // z[i] = f * x[i] + sum_c Q[i][c] w[c] for an FP32 Q with nr % 4 == 0: one
@@ -433,13 +439,13 @@ __global__ __launch_bounds__(256) void qw4_kernel(const float* __restrict__ Q, c
 // (B[kk = h][n = li]); FP32 accumulation over at most 64 row pairs, then FP64.
 // The four waves take interleaved row pairs and are combined in LDS in a fixed
 // order.
-__global__ __launch_bounds__(256) void qtx_mfma_kernel(const float* __restrict__ Q, int64_t n, int nr,
+__device__ __forceinline__ void qtx_mfma_kernel_body(const float* __restrict__ Q, int64_t n, int nr,
                                                        const float* __restrict__ X, int64_t ldx, int k,
-                                                       int64_t rows_per_split, double* __restrict__ part) {
+                                                       int64_t rows_per_split, double* __restrict__ part, int bx_, int by_) {
   __shared__ double red[4][32][33];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 31, h = lane >> 5;
-  const int c0 = blockIdx.x * 32;
-  const int64_t s = blockIdx.y;
+  const int c0 = bx_ * 32;
+  const int64_t s = by_;
   const int64_t r0 = s * rows_per_split;
   const int64_t r1 = (r0 + rows_per_split < n) ? r0 + rows_per_split : n;
   floatx16 acc = {0};
@@ -481,6 +487,12 @@ __global__ __launch_bounds__(256) void qtx_mfma_kernel(const float* __restrict__
   }
 }
 
+__global__ __launch_bounds__(256) void qtx_mfma_kernel(const float* __restrict__ Q, int64_t n, int nr,
+                                                       const float* __restrict__ X, int64_t ldx, int k,
+                                                       int64_t rows_per_split, double* __restrict__ part) {
+  qtx_mfma_kernel_body(Q, n, nr, X, ldx, k, rows_per_split, part, (int)blockIdx.x, (int)blockIdx.y);
+}
+
 // Z[i][j0 + j] = f X[i][j0 + j] + sum_c Q[i][c] W2[c][j0 + j] for j < 16 NB, FP32 Q, on
 // v_mfma_f32_16x16x4f32.  The block stages W2's columns j0 .. j0 + 16 NB - 1 in LDS
 // as FP32, transposed (row j, stride P floats, zero past nr), and each wave owns
@@ -497,10 +509,10 @@ static inline int qw_stride(int nr) {
   return nr16 + (((4 - nr16) % 32) + 32) % 32;  // P = 4 (mod 32): conflict-free 16-byte reads
 }
 template <int NB>
-__global__ __launch_bounds__(256) void qw_mfma_kernel(const float* __restrict__ Q, const double* __restrict__ W2,
+__device__ __forceinline__ void qw_mfma_kernel_body(const float* __restrict__ Q, const double* __restrict__ W2,
                                                       int64_t n, int nr, int P, int k, int j0,
                                                       const float* __restrict__ X, int64_t ldx,
-                                                      float* __restrict__ Z, int64_t ldz, double f) {
+                                                      float* __restrict__ Z, int64_t ldz, double f, int bx_, int by_) {
   extern __shared__ float w2t[];  // 16 NB rows x P
   constexpr int JW = 16 * NB;
   for (int e = threadIdx.x; e < JW * P; e += blockDim.x) {
@@ -509,7 +521,7 @@ __global__ __launch_bounds__(256) void qw_mfma_kernel(const float* __restrict__ 
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-  const int64_t i0 = (int64_t)blockIdx.x * QW_BLOCK_ROWS + (threadIdx.x >> 6) * (16 * QW_TILES);
+  const int64_t i0 = (int64_t)bx_ * QW_BLOCK_ROWS + (threadIdx.x >> 6) * (16 * QW_TILES);
   if (i0 >= n) return;
   const float* qrow[QW_TILES];
 #pragma unroll
@@ -549,6 +561,14 @@ __global__ __launch_bounds__(256) void qw_mfma_kernel(const float* __restrict__ 
         const int j = j0 + 16 * nb + r;
         if (i < n && j < k) Z[i * ldz + j] = (float)(f * (double)X[i * ldx + j] + (double)acc[tt][nb][v]);
       }
+}
+
+template <int NB>
+__global__ __launch_bounds__(256) void qw_mfma_kernel(const float* __restrict__ Q, const double* __restrict__ W2,
+                                                      int64_t n, int nr, int P, int k, int j0,
+                                                      const float* __restrict__ X, int64_t ldx,
+                                                      float* __restrict__ Z, int64_t ldz, double f) {
+  qw_mfma_kernel_body<NB>(Q, W2, n, nr, P, k, j0, X, ldx, Z, ldz, f, (int)blockIdx.x, (int)blockIdx.y);
 }
 
 // out[c] = sum_b part[b][c]; block (64 columns x 4 row groups), fixed order per group
@@ -1323,6 +1343,99 @@ static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, i
   return LAKER_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// Batched launches of the multi-RHS apply's kernels (see batch.h).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void qtx_mfma_batched(const BQtx* __restrict__ desc, const int* __restrict__ off,
+                                                        int nsys) {
+  const int s = batch_system(off, nsys, (int)blockIdx.x);
+  const BQtx d = desc[s];
+  const int lb = (int)blockIdx.x - off[s];
+  qtx_mfma_kernel_body(d.Q, d.n, d.nr, d.X, d.ldx, d.k, d.rps, d.part, lb % d.gx, lb / d.gx);
+}
+
+__global__ __launch_bounds__(256) void reduce_parts_batched(const BRedParts* __restrict__ desc,
+                                                            const int* __restrict__ off, int nsys) {
+  const int s = batch_system(off, nsys, (int)blockIdx.x);
+  const BRedParts d = desc[s];
+  const int lb = (int)blockIdx.x - off[s];
+  reduce_parts_groups_kernel_body(d.part, d.nb, d.width, d.out, lb % d.gx, lb / d.gx, d.gy);
+}
+
+template <int NB>
+__global__ __launch_bounds__(256) void qw_mfma_batched(const BQw* __restrict__ desc, const int* __restrict__ off,
+                                                       int nsys) {
+  const int s = batch_system(off, nsys, (int)blockIdx.x);
+  const BQw d = desc[s];
+  qw_mfma_kernel_body<NB>(d.Q, d.W2, d.n, d.nr, d.P, d.k, d.j0, d.X, d.ldx, d.Z, d.ldz, d.f,
+                          (int)blockIdx.x - off[s], 0);
+}
+
+int batched_qtx(const BatchLaunch<BQtx>& L, hipStream_t st) {
+  if (L.nsys <= 0) return LAKER_OK;
+  hipLaunchKernelGGL(qtx_mfma_batched, dim3((unsigned)L.blocks), dim3(256), 0, st, L.desc, L.off, L.nsys);
+  LAKER_KCHECK("qtx_mfma_batched");
+  return LAKER_OK;
+}
+
+int batched_reduce_parts(const BatchLaunch<BRedParts>& L, hipStream_t st) {
+  if (L.nsys <= 0) return LAKER_OK;
+  hipLaunchKernelGGL(reduce_parts_batched, dim3((unsigned)L.blocks), dim3(256), 0, st, L.desc, L.off, L.nsys);
+  LAKER_KCHECK("reduce_parts_batched");
+  return LAKER_OK;
+}
+
+int batched_qw(const BatchLaunch<BQw>& L, int nb, int P, hipStream_t st) {
+  if (L.nsys <= 0) return LAKER_OK;
+  const size_t lds = (size_t)16 * nb * P * sizeof(float);
+  if (nb == 2)
+    hipLaunchKernelGGL((qw_mfma_batched<2>), dim3((unsigned)L.blocks), dim3(256), lds, st, L.desc, L.off, L.nsys);
+  else
+    hipLaunchKernelGGL((qw_mfma_batched<1>), dim3((unsigned)L.blocks), dim3(256), lds, st, L.desc, L.off, L.nsys);
+  LAKER_KCHECK("qw_mfma_batched");
+  return LAKER_OK;
+}
+
+int qw_block_rows() { return QW_BLOCK_ROWS; }
+
+int precond_batch_view(Precond& pc, int k, BatchApplyView* v) {
+  const int64_t n = pc.n;
+  const int nr = pc.nr;
+  if (pc.kind != PRECOND_CCCP || pc.dtype != LAKER_F32 || k < 2 || k > 32 || nr % 4 != 0 ||
+      (size_t)16 * qw_stride(nr) * sizeof(float) > 64 * 1024)
+    return 1;
+  // the geometry of apply_t's multi-RHS MFMA branch
+  const int cblk = (int)ceil_div(nr, 32);
+  const int64_t nsplit = std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 256), ceil_div(1024, cblk)));
+  const int64_t rps = ceil_div(n, nsplit);
+  const int64_t ns_eff = ceil_div(n, rps);
+  constexpr int RG = 8;
+  const int64_t width = (int64_t)nr * k;
+  LAKER_TRY(pc.w1.ensure((size_t)nr * k * 8));
+  LAKER_TRY(pc.w2.ensure((size_t)nr * k * 8));
+  LAKER_TRY(pc.part.ensure(((size_t)ns_eff + RG) * width * 8));
+  const int P = qw_stride(nr);
+  v->Q = pc.Q.as<float>();
+  v->S = pc.S.as<double>();
+  v->W1 = pc.w1.as<double>();
+  v->W2 = pc.w2.as<double>();
+  v->part = pc.part.as<double>();
+  v->part2 = v->part + (size_t)ns_eff * width;
+  v->n = n;
+  v->rps = rps;
+  v->nr = nr;
+  v->cblk = cblk;
+  v->ns_eff = (int)ns_eff;
+  v->width = (int)width;
+  v->P = P;
+  v->qw_blocks = (int)ceil_div(n, QW_BLOCK_ROWS);
+  // one pass: qw_mfma_kernel<2> for k > 16 (when its LDS fits), else <1> (k <= 16)
+  v->qw_nb = (k > 16 && (size_t)32 * P * sizeof(float) <= 64 * 1024) ? 2 : 1;
+  if (v->qw_nb == 1 && k > 16) return 1;
+  v->f = pc.inv_sqrt_a;
+  return LAKER_OK;
+}
 
 }  // namespace laker
 

@@ -8,6 +8,7 @@
 #include "laker_common.h"
 #include "laker_internal.h"
 #include "dense64.h"
+#include "batch.h"
 
 namespace laker {
 
@@ -82,15 +83,15 @@ __device__ inline void gemm_tile(const TA* __restrict__ A, int64_t lda, const TB
 
 // grid: (ceil(N/64), ceil(M/64), ksplit).  ksplit == 1 writes C directly.
 template <class TA, class TB, class TC, bool TRA, bool TRB>
-__global__ __launch_bounds__(256) void gemm_kernel(int64_t M, int64_t N, int64_t K, double alpha,
+__device__ __forceinline__ void gemm_kernel_body(int64_t M, int64_t N, int64_t K, double alpha,
                                                    const TA* __restrict__ A, int64_t lda,
                                                    const TB* __restrict__ B, int64_t ldb,
                                                    double beta, TC* __restrict__ C, int64_t ldc,
-                                                   double* __restrict__ part, int64_t kchunk) {
+                                                   double* __restrict__ part, int64_t kchunk, int bx_, int by_, int bz_) {
   __shared__ double As[GBK][GBM + 1];
   __shared__ double Bs[GBK][GBN + 1];
-  const int64_t m0 = (int64_t)blockIdx.y * GBM, n0 = (int64_t)blockIdx.x * GBN;
-  const int64_t k0 = (int64_t)blockIdx.z * kchunk;
+  const int64_t m0 = (int64_t)by_ * GBM, n0 = (int64_t)bx_ * GBN;
+  const int64_t k0 = (int64_t)bz_ * kchunk;
   const int64_t k1 = (k0 + kchunk < K) ? k0 + kchunk : K;
   doublex4 acc[2][2];
 #pragma unroll
@@ -109,7 +110,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(int64_t M, int64_t N, int64_t
         const int64_t gn = n0 + 32 * wn + 16 * j + (lane & 15);
         if (gm < M && gn < N) {
           if (part) {
-            part[((int64_t)blockIdx.z * M + gm) * N + gn] = acc[i][j][r];
+            part[((int64_t)bz_ * M + gm) * N + gn] = acc[i][j][r];
           } else {
             double v = alpha * acc[i][j][r];
             if (beta != 0.0) v += beta * (double)C[gm * ldc + gn];
@@ -117,6 +118,15 @@ __global__ __launch_bounds__(256) void gemm_kernel(int64_t M, int64_t N, int64_t
           }
         }
       }
+}
+
+template <class TA, class TB, class TC, bool TRA, bool TRB>
+__global__ __launch_bounds__(256) void gemm_kernel(int64_t M, int64_t N, int64_t K, double alpha,
+                                                   const TA* __restrict__ A, int64_t lda,
+                                                   const TB* __restrict__ B, int64_t ldb,
+                                                   double beta, TC* __restrict__ C, int64_t ldc,
+                                                   double* __restrict__ part, int64_t kchunk) {
+  gemm_kernel_body<TA, TB, TC, TRA, TRB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, part, kchunk, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
 }
 
 template <class TC>
@@ -359,4 +369,25 @@ int set_identity(double* A, int64_t n, double diag, hipStream_t st) {
   return LAKER_OK;
 }
 
+// Batched FP64 C = alpha A B + beta C (no transposes, no split-K), see batch.h.
+__global__ __launch_bounds__(256) void gemm_batched(const BGemm* __restrict__ desc, const int* __restrict__ off,
+                                                    int nsys) {
+  const int s = batch_system(off, nsys, (int)blockIdx.x);
+  const BGemm d = desc[s];
+  const int lb = (int)blockIdx.x - off[s];
+  const int64_t kk = d.K > 1 ? d.K : 1;
+  const int64_t kchunk = (kk + GBK - 1) / GBK * GBK;  // one K chunk (no split-K)
+  gemm_kernel_body<double, double, double, false, false>(d.M, d.N, d.K, d.alpha, d.A, d.lda, d.B, d.ldb, d.beta,
+                                                         d.C, d.ldc, (double*)nullptr, kchunk, lb % d.gx,
+                                                         lb / d.gx, 0);
+}
+
+int batched_gemm(const BatchLaunch<BGemm>& L, hipStream_t st) {
+  if (L.nsys <= 0) return LAKER_OK;
+  hipLaunchKernelGGL(gemm_batched, dim3((unsigned)L.blocks), dim3(256), 0, st, L.desc, L.off, L.nsys);
+  LAKER_KCHECK("gemm_batched");
+  return LAKER_OK;
+}
+
 }  // namespace laker
+

@@ -1,6 +1,7 @@
 // Multi-RHS products on MFMA: mv_multi_kernel (FP32 right-hand sides),
 // pack_x_kernel + mv_multi2_kernel (BF16-split right-hand sides).
 #include "matvec_tiles.h"
+#include "batch.h"
 
 #include <algorithm>
 #include <cmath>
@@ -130,10 +131,10 @@ __global__ __launch_bounds__(256) void mv_multi_kernel(
 // Xp index: ((t * nq + q) * 2 + u) * 64 + lane, 8 bf16 values
 //   X[32 t + jj(8u + v, lane/32)][c_base + 32 q + lane%32], v = 0..7.
 // ---------------------------------------------------------------------------
-__global__ void pack_x_kernel(const float* __restrict__ X, int64_t ldx, int64_t n_cols, int kcols,
+__device__ __forceinline__ void pack_x_kernel_body(const float* __restrict__ X, int64_t ldx, int64_t n_cols, int kcols,
                               int c_base, int64_t ntiles, int nq, bf16x8* __restrict__ xh,
-                              bf16x8* __restrict__ xl, bf16x8* __restrict__ xm) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                              bf16x8* __restrict__ xl, bf16x8* __restrict__ xm, int bx_, int by_) {
+  const int64_t idx = (int64_t)bx_ * blockDim.x + threadIdx.x;
   if (idx >= ntiles * 128 * nq) return;
   const int lane = (int)(idx & 63), u = (int)((idx >> 6) & 1);
   const int q = (int)((idx >> 7) % nq);
@@ -158,6 +159,12 @@ __global__ void pack_x_kernel(const float* __restrict__ X, int64_t ldx, int64_t 
   if (xm) xm[idx] = mid;
 }
 
+__global__ void pack_x_kernel(const float* __restrict__ X, int64_t ldx, int64_t n_cols, int kcols,
+                              int c_base, int64_t ntiles, int nq, bf16x8* __restrict__ xh,
+                              bf16x8* __restrict__ xl, bf16x8* __restrict__ xm) {
+  pack_x_kernel_body(X, ldx, n_cols, kcols, c_base, ntiles, nq, xh, xl, xm, (int)blockIdx.x, (int)blockIdx.y);
+}
+
 constexpr int GX_SPLIT3 = 5;  // exp tile and X both split 3-way, 6 BF16 products (~2^-24)
 constexpr int GX_PLXH = 6;    // exp tile split hi/lo, X exactly BF16: 2 products (hi.X + lo.X)
 
@@ -176,11 +183,11 @@ __host__ __device__ constexpr int m2_chunk_bytes(int ns, int nparts, int ncc, in
 // 3-way split (LAKER_PREC_MIXED) or the FP16 2-way split (GRAM_F16S, square
 // operator only).
 template <int GM, int NS, int GX, int NCC, int JT, int ITW, bool CLAMP>
-__global__ __launch_bounds__(256) void mv_multi2_kernel(
+__device__ __forceinline__ void mv_multi2_kernel_body(
     const void* __restrict__ colpack, const void* __restrict__ rowpack, int ka, int64_t n_rows,
     int64_t n_cols, const bf16x8* __restrict__ xh, const bf16x8* __restrict__ xl,
     const bf16x8* __restrict__ xm, int kcols, int c_base, float* __restrict__ part, int64_t ldp,
-    int jsplit, float clampv) {
+    int jsplit, float clampv, int bx_, int by_) {
   using G = GramOps<GM>;
   constexpr int NP = M2Parts<GX>::value;
   constexpr int gb = 32 * NS, lb = gb + 16, vpr = gb / 16;
@@ -191,8 +198,8 @@ __global__ __launch_bounds__(256) void mv_multi2_kernel(
   constexpr int CHUNK = m2_chunk_bytes(NS, NP, NCC, JT);
   extern __shared__ __align__(16) char m2_smem[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, h = lane >> 5, li = lane & 31;
-  const int js = blockIdx.x % jsplit;
-  const int64_t ib = blockIdx.x / jsplit;
+  const int js = bx_ % jsplit;
+  const int64_t ib = bx_ / jsplit;
   const int64_t I0 = (ib * 4 + w) * (32 * ITW);
   const bool active = I0 < n_rows;
   (void)ka;
@@ -330,6 +337,15 @@ __global__ __launch_bounds__(256) void mv_multi2_kernel(
         const int c = c_base + q * 32 + li;
         if (i < n_rows && c < kcols) part[((int64_t)js * n_rows + i) * ldp + c] = acc[it][q][r];
       }
+}
+
+template <int GM, int NS, int GX, int NCC, int JT, int ITW, bool CLAMP>
+__global__ __launch_bounds__(256) void mv_multi2_kernel(
+    const void* __restrict__ colpack, const void* __restrict__ rowpack, int ka, int64_t n_rows,
+    int64_t n_cols, const bf16x8* __restrict__ xh, const bf16x8* __restrict__ xl,
+    const bf16x8* __restrict__ xm, int kcols, int c_base, float* __restrict__ part, int64_t ldp,
+    int jsplit, float clampv) {
+  mv_multi2_kernel_body<GM, NS, GX, NCC, JT, ITW, CLAMP>(colpack, rowpack, ka, n_rows, n_cols, xh, xl, xm, kcols, c_base, part, ldp, jsplit, clampv, (int)blockIdx.x, (int)blockIdx.y);
 }
 
 static int launch_multi(int mode, int gx, const void* colpack, const void* rowpack, int ka,
@@ -545,4 +561,87 @@ int product_multi_f32(Operator& op, int precision, const Side& rows, const Side&
   return LAKER_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Batched launches (several systems per launch, see batch.h): the pack, the
+// GX_SPLIT3 multi-RHS product and its reduction, with each system's block
+// decomposition exactly that of product_multi_f32 for one 32-column chunk.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void pack_x_batched(const BPackX* __restrict__ desc, const int* __restrict__ off,
+                                                      int nsys) {
+  const int s = batch_system(off, nsys, (int)blockIdx.x);
+  const BPackX d = desc[s];
+  pack_x_kernel_body(d.X, d.ldx, d.n_cols, d.kcols, d.c_base, d.ntiles, d.nq, d.xh, d.xl, d.xm,
+                     (int)blockIdx.x - off[s], 0);
+}
+
+template <int GM, int NS, int JT, bool CLAMP>
+__global__ __launch_bounds__(256) void mv_multi2_batched(const BMulti2* __restrict__ desc, const int* __restrict__ off,
+                                                         int nsys) {
+  const int s = batch_system(off, nsys, (int)blockIdx.x);
+  const BMulti2 d = desc[s];
+  mv_multi2_kernel_body<GM, NS, GX_SPLIT3, 1, JT, 1, CLAMP>(d.colpack, d.rowpack, d.ka, d.n_rows, d.n_cols, d.xh,
+                                                             d.xl, d.xm, d.kcols, d.c_base, d.part, d.ldp, d.jsplit,
+                                                             d.clampv, (int)blockIdx.x - off[s], 0);
+}
+
+__global__ void reduce_batched(const BReduce* __restrict__ desc, const int* __restrict__ off, int nsys) {
+  const int s = batch_system(off, nsys, (int)blockIdx.x);
+  const BReduce d = desc[s];
+  mv_reduce_kernel_body<float, float, float, false>(d.part, d.n_rows, d.k, d.ldp, d.jsplit, d.lam, d.Xr, d.ldx,
+                                                    (const float*)nullptr, d.Y, d.ldy, 0, (const int*)nullptr,
+                                                    (int)blockIdx.x - off[s], 0);
+}
+
+int multi2_split3_jt(int /*gm*/, int ns) { return 2 * m2_chunk_bytes(ns, 3, 1, 2) <= 65536 ? 2 : 1; }
+size_t multi2_split3_lds(int ns, int jt) { return 2 * (size_t)m2_chunk_bytes(ns, 3, 1, jt); }
+
+int batched_pack_x(const BatchLaunch<BPackX>& L, hipStream_t st) {
+  if (L.nsys <= 0) return LAKER_OK;
+  hipLaunchKernelGGL(pack_x_batched, dim3((unsigned)L.blocks), dim3(256), 0, st, L.desc, L.off, L.nsys);
+  LAKER_KCHECK("pack_x_batched");
+  return LAKER_OK;
+}
+
+int batched_multi2(const BatchLaunch<BMulti2>& L, int gm, int ns, bool clamp, hipStream_t st) {
+  if (L.nsys <= 0) return LAKER_OK;
+  const int jt = multi2_split3_jt(gm, ns);
+  const size_t lds = multi2_split3_lds(ns, jt);
+  const dim3 grid((unsigned)L.blocks), block(256);
+#define LAKER_B2(GM_, NS_, JT_)                                                                          \
+  do {                                                                                                 \
+    if (clamp)                                                                                         \
+      hipLaunchKernelGGL((mv_multi2_batched<GM_, NS_, JT_, true>), grid, block, lds, st, L.desc, L.off, L.nsys); \
+    else                                                                                               \
+      hipLaunchKernelGGL((mv_multi2_batched<GM_, NS_, JT_, false>), grid, block, lds, st, L.desc, L.off, L.nsys); \
+  } while (0)
+#define LAKER_B2J(GM_, NS_)                 \
+  do {                                      \
+    if (jt == 2) LAKER_B2(GM_, NS_, 2);     \
+    else LAKER_B2(GM_, NS_, 1);             \
+  } while (0)
+  if (gm == GRAM_F16S) {
+    if (ns == 2) LAKER_B2J(GRAM_F16S, 2);
+    else if (ns == 3) LAKER_B2J(GRAM_F16S, 3);
+    else return fail(LAKER_EINVAL, "batched product: no FP16-split instance for this embedding width");
+  } else {
+    if (ns == 2) LAKER_B2J(LAKER_PREC_MIXED, 2);
+    else if (ns == 3) LAKER_B2J(LAKER_PREC_MIXED, 3);
+    else if (ns == 4) LAKER_B2J(LAKER_PREC_MIXED, 4);
+    else if (ns == 6) LAKER_B2J(LAKER_PREC_MIXED, 6);
+    else return fail(LAKER_EINVAL, "batched product: no BF16-split instance for this embedding width");
+  }
+#undef LAKER_B2J
+#undef LAKER_B2
+  LAKER_KCHECK("mv_multi2_batched");
+  return LAKER_OK;
+}
+
+int batched_reduce(const BatchLaunch<BReduce>& L, hipStream_t st) {
+  if (L.nsys <= 0) return LAKER_OK;
+  hipLaunchKernelGGL(reduce_batched, dim3((unsigned)L.blocks), dim3(256), 0, st, L.desc, L.off, L.nsys);
+  LAKER_KCHECK("reduce_batched");
+  return LAKER_OK;
+}
+
 }  // namespace laker
+

@@ -176,12 +176,12 @@ __host__ __device__ inline int64_t s64_coff(int64_t b, int64_t n) { return b * n
 
 // MODE_RES: Y = B - lam*Xr - sum   (FP64 residual)
 template <class PT, class XT, class YT, bool RES>
-__global__ void mv_reduce_kernel(const PT* __restrict__ part, int64_t n_rows, int k, int64_t ldp,
+__device__ __forceinline__ void mv_reduce_kernel_body(const PT* __restrict__ part, int64_t n_rows, int k, int64_t ldp,
                                  int jsplit, double lam, const XT* __restrict__ Xr, int64_t ldx,
                                  const YT* __restrict__ B, YT* __restrict__ Y, int64_t ldy,
-                                 int accumulate, const int* __restrict__ skip) {
+                                 int accumulate, const int* __restrict__ skip, int bx_, int by_) {
   if (skip && *skip) return;
-  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t idx = (int64_t)bx_ * blockDim.x + threadIdx.x;
   if (idx >= n_rows * k) return;
   const int64_t i = idx / k;
   const int c = (int)(idx % k);
@@ -194,6 +194,14 @@ __global__ void mv_reduce_kernel(const PT* __restrict__ part, int64_t n_rows, in
     if (accumulate) s += (double)Y[i * ldy + c];
     Y[i * ldy + c] = (YT)s;
   }
+}
+
+template <class PT, class XT, class YT, bool RES>
+__global__ void mv_reduce_kernel(const PT* __restrict__ part, int64_t n_rows, int k, int64_t ldp,
+                                 int jsplit, double lam, const XT* __restrict__ Xr, int64_t ldx,
+                                 const YT* __restrict__ B, YT* __restrict__ Y, int64_t ldy,
+                                 int accumulate, const int* __restrict__ skip) {
+  mv_reduce_kernel_body<PT, XT, YT, RES>(part, n_rows, k, ldp, jsplit, lam, Xr, ldx, B, Y, ldy, accumulate, skip, (int)blockIdx.x, (int)blockIdx.y);
 }
 
 // Column splits of a split-K product: enough workgroups to fill the chip.

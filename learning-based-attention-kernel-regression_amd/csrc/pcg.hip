@@ -11,12 +11,16 @@
 // status word every `check_every` iterations, and every kernel after the
 // stopping point is a no-op.
 #include <atomic>
+#include <cstring>
+#include <memory>
 #include <cmath>
 #include <vector>
 
 #include "dense64.h"
 #include "laker_common.h"
 #include "laker_internal.h"
+#include "batch.h"
+#include "matvec_tiles.h"
 
 namespace laker {
 
@@ -49,15 +53,15 @@ constexpr int DOT_NB = 512;  // max partial blocks
 
 // part[q][blk][c] = sum over the block's rows of A_q[i][c] * B_q[i][c]
 template <class T, int NQ>
-__global__ __launch_bounds__(256) void coldots_kernel(const T* __restrict__ A0, const T* __restrict__ B0,
+__device__ __forceinline__ void coldots_kernel_body(const T* __restrict__ A0, const T* __restrict__ B0,
                                                       const T* __restrict__ A1, const T* __restrict__ B1,
                                                       const T* __restrict__ A2, const T* __restrict__ B2,
                                                       int64_t n, int k, int64_t rpb,
                                                       double* __restrict__ part, int nblk,
-                                                      const PcgState* __restrict__ st) {
+                                                      const PcgState* __restrict__ st, int bx_, int by_) {
   if (st && st->done) return;
   __shared__ double sm[NQ][256];
-  const int64_t r0 = (int64_t)blockIdx.x * rpb;
+  const int64_t r0 = (int64_t)bx_ * rpb;
   const int64_t r1 = (r0 + rpb < n) ? r0 + rpb : n;
   if (k == 1) {
     double acc[NQ];
@@ -72,7 +76,7 @@ __global__ __launch_bounds__(256) void coldots_kernel(const T* __restrict__ A0, 
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const double s = block_sum(acc[q], red);
-      if (threadIdx.x == 0) part[(int64_t)q * nblk + blockIdx.x] = s;
+      if (threadIdx.x == 0) part[(int64_t)q * nblk + bx_] = s;
     }
     return;
   }
@@ -98,10 +102,20 @@ __global__ __launch_bounds__(256) void coldots_kernel(const T* __restrict__ A0, 
 #pragma unroll
       for (int q = 0; q < NQ; ++q) {
         const double s = sm[q][tc] + sm[q][tc + 64] + sm[q][tc + 128] + sm[q][tc + 192];
-        part[((int64_t)q * nblk + blockIdx.x) * k + c] = s;
+        part[((int64_t)q * nblk + bx_) * k + c] = s;
       }
     }
   }
+}
+
+template <class T, int NQ>
+__global__ __launch_bounds__(256) void coldots_kernel(const T* __restrict__ A0, const T* __restrict__ B0,
+                                                      const T* __restrict__ A1, const T* __restrict__ B1,
+                                                      const T* __restrict__ A2, const T* __restrict__ B2,
+                                                      int64_t n, int k, int64_t rpb,
+                                                      double* __restrict__ part, int nblk,
+                                                      const PcgState* __restrict__ st) {
+  coldots_kernel_body<T, NQ>(A0, B0, A1, B1, A2, B2, n, k, rpb, part, nblk, st, (int)blockIdx.x, (int)blockIdx.y);
 }
 
 // out[q][c] = sum_blk part[q][blk][c], one wave per column in a fixed order:
@@ -115,9 +129,9 @@ __device__ inline double wave_sum_part(const double* part, int q, int nblk, int 
 }
 
 // alpha = rz / pAp with the breakdown test.  One block, one wave per column.
-__global__ void pcg_alpha_kernel(const double* __restrict__ part, int nblk, int k,
+__device__ __forceinline__ void pcg_alpha_kernel_body(const double* __restrict__ part, int nblk, int k,
                                  const double* __restrict__ rz, double* __restrict__ alpha,
-                                 PcgState* __restrict__ st) {
+                                 PcgState* __restrict__ st, int bx_, int by_) {
   if (st->done) return;
   __shared__ int brk;
   if (threadIdx.x == 0) brk = 0;
@@ -137,6 +151,12 @@ __global__ void pcg_alpha_kernel(const double* __restrict__ part, int nblk, int 
     st->done = PCG_BREAKDOWN;
     st->iterations = st->it;  // iterations completed before the breakdown
   }
+}
+
+__global__ void pcg_alpha_kernel(const double* __restrict__ part, int nblk, int k,
+                                 const double* __restrict__ rz, double* __restrict__ alpha,
+                                 PcgState* __restrict__ st) {
+  pcg_alpha_kernel_body(part, nblk, k, rz, alpha, st, (int)blockIdx.x, (int)blockIdx.y);
 }
 
 // x += alpha p ; r -= alpha Ap
@@ -235,8 +255,8 @@ __global__ __launch_bounds__(256) void pcg_update_xr_rr_alpha_kernel(T* __restri
 }
 
 // residual norm, convergence and max-iteration bookkeeping.  One block.
-__global__ void pcg_resid_kernel(const double* __restrict__ part, int nblk, int k,
-                                 PcgState* __restrict__ st) {
+__device__ __forceinline__ void pcg_resid_kernel_body(const double* __restrict__ part, int nblk, int k,
+                                 PcgState* __restrict__ st, int bx_, int by_) {
   if (st->done) return;
   const int it = st->it, max_iter = st->max_iter;
   __shared__ double red[16];
@@ -256,6 +276,11 @@ __global__ void pcg_resid_kernel(const double* __restrict__ part, int nblk, int 
     }
     st->it = it + 1;
   }
+}
+
+__global__ void pcg_resid_kernel(const double* __restrict__ part, int nblk, int k,
+                                 PcgState* __restrict__ st) {
+  pcg_resid_kernel_body(part, nblk, k, st, (int)blockIdx.x, (int)blockIdx.y);
 }
 
 // k = 1: the bookkeeping of pcg_resid_kernel (from the ||r||^2 partials rr of
@@ -307,8 +332,8 @@ __global__ __launch_bounds__(256) void pcg_resid_rz_kernel(const double* __restr
 
 // beta = rz_new / rz_old ; rz_old = rz_new.  One block.
 // bump_it: also advance the iteration counter (after pcg_resid_rz_kernel).
-__global__ void pcg_beta_kernel(const double* __restrict__ part, int nblk, int k, double* __restrict__ rz,
-                                double* __restrict__ beta, PcgState* __restrict__ st, int bump_it) {
+__device__ __forceinline__ void pcg_beta_kernel_body(const double* __restrict__ part, int nblk, int k, double* __restrict__ rz,
+                                double* __restrict__ beta, PcgState* __restrict__ st, int bump_it, int bx_, int by_) {
   if (st->done) return;
   if (bump_it && threadIdx.x == 0) st->it += 1;
   const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
@@ -321,16 +346,28 @@ __global__ void pcg_beta_kernel(const double* __restrict__ part, int nblk, int k
   }
 }
 
+__global__ void pcg_beta_kernel(const double* __restrict__ part, int nblk, int k, double* __restrict__ rz,
+                                double* __restrict__ beta, PcgState* __restrict__ st, int bump_it) {
+  pcg_beta_kernel_body(part, nblk, k, rz, beta, st, bump_it, (int)blockIdx.x, (int)blockIdx.y);
+}
+
 // p = z + beta p
+template <class T>
+__device__ __forceinline__ void pcg_update_p_kernel_body(T* __restrict__ p, const T* __restrict__ z,
+                                    const double* __restrict__ beta, int64_t n, int k,
+                                    const PcgState* __restrict__ st, int bx_, int by_) {
+  if (st->done) return;
+  const int64_t idx = (int64_t)bx_ * blockDim.x + threadIdx.x;
+  if (idx >= n * k) return;
+  const int c = (int)(idx % k);
+  p[idx] = (T)((double)z[idx] + beta[c] * (double)p[idx]);
+}
+
 template <class T>
 __global__ void pcg_update_p_kernel(T* __restrict__ p, const T* __restrict__ z,
                                     const double* __restrict__ beta, int64_t n, int k,
                                     const PcgState* __restrict__ st) {
-  if (st->done) return;
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n * k) return;
-  const int c = (int)(idx % k);
-  p[idx] = (T)((double)z[idx] + beta[c] * (double)p[idx]);
+  pcg_update_p_kernel_body<T>(p, z, beta, n, k, st, (int)blockIdx.x, (int)blockIdx.y);
 }
 
 // set-up scalars: rz = colsum(r z), rhs_norm = ||b||_F.  One block.
@@ -742,12 +779,12 @@ __global__ __launch_bounds__(256) void pcg_update_x64_rr_alpha_kernel(double* __
 }
 
 // x (FP64) += f * alpha p ; r -= alpha Ap  (FP32 recursion on a scaled residual)
-__global__ void pcg_update_x64_kernel(double* __restrict__ x, float* __restrict__ r,
+__device__ __forceinline__ void pcg_update_x64_kernel_body(double* __restrict__ x, float* __restrict__ r,
                                       const float* __restrict__ p, const float* __restrict__ ap,
                                       const double* __restrict__ alpha, int64_t n, int k,
-                                      const PcgState* __restrict__ st) {
+                                      const PcgState* __restrict__ st, int bx_, int by_) {
   if (st->done) return;
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t idx = (int64_t)bx_ * blockDim.x + threadIdx.x;
   if (idx >= n * k) return;
   const double f = st->xscale;
   const double a = alpha[idx % k];
@@ -755,7 +792,385 @@ __global__ void pcg_update_x64_kernel(double* __restrict__ x, float* __restrict_
   r[idx] = (float)((double)r[idx] - a * (double)ap[idx]);
 }
 
+__global__ void pcg_update_x64_kernel(double* __restrict__ x, float* __restrict__ r,
+                                      const float* __restrict__ p, const float* __restrict__ ap,
+                                      const double* __restrict__ alpha, int64_t n, int k,
+                                      const PcgState* __restrict__ st) {
+  pcg_update_x64_kernel_body(x, r, p, ap, alpha, n, k, st, (int)blockIdx.x, (int)blockIdx.y);
+}
 
+
+
+
+// ---------------------------------------------------------------------------
+// Batched mixed-precision solves (laker_batch_*): the inner PCG iterations of
+// many small systems (uneven n, 2 <= k <= 32 right-hand sides each) run as
+// shared launches -- one launch per kernel of the iteration for all systems
+// whose refinement step is running -- and each system's refinement steps
+// (FP64 residual, best iterate, next step) are taken on the host between
+// polls, exactly as laker_solve_mixed takes them.  Every system keeps its
+// own operator, preconditioner and Krylov buffers, its kernels run with the
+// block decomposition of an unbatched launch, so a system's iterates do not
+// depend on which other systems share the launches.
+// ---------------------------------------------------------------------------
+template <int NQ>
+__global__ __launch_bounds__(256) void coldots_batched(const BColdots* __restrict__ desc, const int* __restrict__ off,
+                                                       int nsys) {
+  const int s = batch_system(off, nsys, (int)blockIdx.x);
+  const BColdots d = desc[s];
+  coldots_kernel_body<float, NQ>(d.A0, d.B0, d.A1, d.B1, d.A2, d.B2, d.n, d.k, d.rpb, d.part, d.nblk, d.st,
+                                 (int)blockIdx.x - off[s], 0);
+}
+__global__ void alpha_batched(const BScalar* __restrict__ desc, int nsys) {
+  const BScalar d = desc[blockIdx.x];
+  pcg_alpha_kernel_body(d.part, d.nblk, d.k, d.rz, d.out, d.st, 0, 0);
+}
+__global__ void resid_batched(const BScalar* __restrict__ desc, int nsys) {
+  const BScalar d = desc[blockIdx.x];
+  pcg_resid_kernel_body(d.part, d.nblk, d.k, d.st, 0, 0);
+}
+__global__ void beta_batched(const BScalar* __restrict__ desc, int nsys) {
+  const BScalar d = desc[blockIdx.x];
+  pcg_beta_kernel_body(d.part, d.nblk, d.k, d.rz, d.out, d.st, 0, 0, 0);
+}
+__global__ void update_x64_batched(const BUpdX64* __restrict__ desc, const int* __restrict__ off, int nsys) {
+  const int s = batch_system(off, nsys, (int)blockIdx.x);
+  const BUpdX64 d = desc[s];
+  pcg_update_x64_kernel_body(d.x, d.r, d.p, d.ap, d.alpha, d.n, d.k, d.st, (int)blockIdx.x - off[s], 0);
+}
+__global__ void update_p_batched(const BUpdP* __restrict__ desc, const int* __restrict__ off, int nsys) {
+  const int s = batch_system(off, nsys, (int)blockIdx.x);
+  const BUpdP d = desc[s];
+  pcg_update_p_kernel_body<float>(d.p, d.z, d.beta, d.n, d.k, d.st, (int)blockIdx.x - off[s], 0);
+}
+
+struct BatchSys {
+  Operator* op = nullptr;
+  Precond* pc = nullptr;
+  const double* B = nullptr;
+  double* X = nullptr;
+  int k = 0;
+  int64_t n = 0;
+  double tol = 1e-6, inner_tol = 5e-2;
+  int max_outer = 20, inner_max_iter = 1000;
+  PcgBuffers<float> b;
+  double *R = nullptr, *Xbest = nullptr, *Rbest = nullptr;
+  DevBuf tmp;
+  const double* Rin = nullptr;
+  double bnorm = 0.0, rnorm = 0.0, rbest = 0.0;
+  bool have_best = false, final_step = false, running = false, finished = false;
+  int it = 0, seg_start = 0;
+  laker_mixed_result res{};
+  // product geometry (product_multi_f32 for k <= 32 on the GX_SPLIT3 path)
+  int gm = 0, ns = 0, jsplit = 1;
+  bool clamp = false;
+  const void *colpack = nullptr, *rowpack = nullptr;
+  int ka = 0;
+  bf16x8 *xh = nullptr, *xl = nullptr, *xm = nullptr;
+  int64_t ntiles = 0;
+  BatchApplyView av{};
+};
+
+struct Batch {
+  hipStream_t st = nullptr;
+  int max_sys = 0;
+  std::vector<BatchSys*> slot;
+  DevBuf states;    // PcgState per slot
+  DevBuf table_d;   // descriptor tables and block offsets of one poll
+  void* table_h = nullptr;  // pinned staging copy
+  size_t table_bytes = 0;
+  std::vector<PcgState> hs;
+  ~Batch() {
+    for (BatchSys* s : slot) delete s;
+    if (table_h) (void)hipHostFree(table_h);
+  }
+};
+
+// one FP64 refinement residual of system s and the bookkeeping of
+// laker_solve_mixed after an inner step; starts the next step or finishes
+static int batch_step_begin(Batch& bt, BatchSys& s);
+
+static int batch_finish_step(Batch& bt, BatchSys& s, const PcgState& h) {
+  const hipStream_t st = bt.st;
+  const int64_t nk = s.n * s.k;
+  s.running = false;
+  s.it = h.done != PCG_RUNNING ? h.iterations : h.it;
+  s.res.inner_iterations = s.it;
+  if (h.done == PCG_BREAKDOWN && s.it == s.seg_start) {
+    s.res.status = LAKER_EBREAKDOWN;
+    s.finished = true;
+    return LAKER_OK;
+  }
+  const bool clamp = s.op->clamp_for(LAKER_CLAMP_AUTO);
+  LAKER_TRY(fused_product_f64<double>(*s.op, s.op->rows(), s.op->cols(), s.X, s.k, s.k, s.R, s.k, s.op->lambda_reg,
+                                      s.X, s.k, clamp, s.B, st));
+  LAKER_TRY(frob_norm(s.R, nk, s.tmp, &s.rnorm, st));
+  s.res.outer_iterations += 1;
+  bool stop = false;
+  if (s.rnorm / s.bnorm <= s.tol) {
+    s.res.converged = 1;
+    s.res.residual_norm = s.rnorm;
+    stop = true;
+  } else if (std::isfinite(s.rnorm) && s.rnorm < s.rbest) {
+    s.rbest = s.rnorm;
+    s.have_best = true;
+    LAKER_HIP_CHECK(hipMemcpyAsync(s.Xbest, s.X, nk * 8, hipMemcpyDeviceToDevice, st));
+    LAKER_HIP_CHECK(hipMemcpyAsync(s.Rbest, s.R, nk * 8, hipMemcpyDeviceToDevice, st));
+    if (s.final_step) stop = true;
+  } else {
+    if (s.have_best) {
+      LAKER_HIP_CHECK(hipMemcpyAsync(s.X, s.Xbest, nk * 8, hipMemcpyDeviceToDevice, st));
+      LAKER_HIP_CHECK(hipMemcpyAsync(s.R, s.Rbest, nk * 8, hipMemcpyDeviceToDevice, st));
+    } else {
+      LAKER_HIP_CHECK(hipMemsetAsync(s.X, 0, nk * 8, st));
+      LAKER_HIP_CHECK(hipMemcpyAsync(s.R, s.B, nk * 8, hipMemcpyDeviceToDevice, st));
+    }
+    s.rnorm = s.rbest;
+    if (s.final_step) stop = true;
+    s.final_step = true;
+  }
+  if (!stop) {
+    s.res.residual_norm = s.rnorm;
+    if (s.res.outer_iterations >= s.max_outer || s.it >= s.inner_max_iter || s.rnorm == 0.0) stop = true;
+  }
+  if (stop) {
+    s.res.residual_norm = std::min(s.rnorm, s.rbest);
+    s.res.inner_iterations = s.it;
+    s.finished = true;
+    return LAKER_OK;
+  }
+  s.Rin = s.R;
+  return batch_step_begin(bt, s);
+}
+
+static int batch_step_begin(Batch& bt, BatchSys& s) {
+  const hipStream_t st = bt.st;
+  const int64_t nk = s.n * s.k;
+  const dim3 eg((unsigned)ceil_div(nk, 256)), eb(256);
+  const double need = 0.6 * s.tol * s.bnorm / s.rnorm;
+  const double seg_tol = std::min(0.5, s.final_step ? need : std::max(s.inner_tol, need));
+  const double eps = std::sqrt(1.1920928955078125e-07);
+  hipLaunchKernelGGL(scale_to_f32_kernel, eg, eb, 0, st, s.Rin, s.b.r, nk, 1.0 / s.rnorm);
+  LAKER_KCHECK("scale_to_f32_kernel");
+  LAKER_TRY(precond_apply(*s.pc, s.b.r, s.k, s.k, s.b.z, s.k, st));
+  hipLaunchKernelGGL((copy_kernel<float>), eg, eb, 0, st, (const float*)s.b.z, s.b.p, nk);
+  LAKER_KCHECK("copy_kernel");
+  LAKER_TRY(coldots<float>(s.b, 2, s.b.r, s.b.z, s.b.r, s.b.r, nullptr, nullptr, s.n, s.k, false, st));
+  hipLaunchKernelGGL(pcg_setup_kernel, dim3(1), dim3(256), 0, st, s.b.part, s.b.nblk, s.k, s.b.rz, s.b.st, seg_tol,
+                     eps, s.it, s.inner_max_iter, s.rnorm);
+  LAKER_KCHECK("pcg_setup_kernel");
+  s.seg_start = s.it;
+  s.running = true;
+  return LAKER_OK;
+}
+
+// Descriptor tables of one poll, laid out in one staging buffer.
+struct TableBuilder {
+  std::vector<char> h;
+  size_t add_bytes(const void* p, size_t bytes) {
+    const size_t o = round_up(h.size(), 64);
+    h.resize(o + bytes);
+    memcpy(h.data() + o, p, bytes);
+    return o;
+  }
+  template <class D>
+  struct Part {
+    size_t desc_off = 0, off_off = 0;
+    int nsys = 0, blocks = 0;
+  };
+  template <class D>
+  Part<D> add(const std::vector<D>& d, const std::vector<int>& blocks) {
+    Part<D> p;
+    p.nsys = (int)d.size();
+    std::vector<int> off(d.size() + 1, 0);
+    for (size_t i = 0; i < d.size(); ++i) off[i + 1] = off[i] + blocks[i];
+    p.blocks = off.back();
+    if (!d.empty()) {
+      p.desc_off = add_bytes(d.data(), d.size() * sizeof(D));
+      p.off_off = add_bytes(off.data(), off.size() * sizeof(int));
+    }
+    return p;
+  }
+  template <class D>
+  BatchLaunch<D> launch(const Part<D>& p, const char* base) const {
+    BatchLaunch<D> L;
+    L.desc = reinterpret_cast<const D*>(base + p.desc_off);
+    L.off = reinterpret_cast<const int*>(base + p.off_off);
+    L.nsys = p.nsys;
+    L.blocks = p.blocks;
+    return L;
+  }
+};
+
+// Inner iterations of every running system, `count` times, as shared launches.
+static int batch_iterate(Batch& bt, const std::vector<BatchSys*>& act, int count) {
+  const hipStream_t st = bt.st;
+  TableBuilder tb;
+  // product: groups by Gram instance (gm, ns, clamp)
+  struct MGroup {
+    int gm, ns;
+    bool clamp;
+    std::vector<BMulti2> d;
+    std::vector<int> blocks;
+    TableBuilder::Part<BMulti2> part;
+  };
+  std::vector<MGroup> mg;
+  std::vector<BPackX> px;
+  std::vector<int> pxb;
+  std::vector<BReduce> rd;
+  std::vector<int> rdb;
+  std::vector<BColdots> cd3, cdr, cdz;
+  std::vector<int> cdb;
+  std::vector<BScalar> sa, sr, sb;
+  std::vector<BUpdX64> ux;
+  std::vector<int> uxb;
+  std::vector<BUpdP> up;
+  std::vector<BQtx> qt;
+  std::vector<int> qtb;
+  std::vector<BRedParts> r1, r2;
+  std::vector<int> r1b, r2b;
+  std::vector<BGemm> gm;
+  std::vector<int> gmb;
+  struct QGroup {
+    int nb, P;
+    std::vector<BQw> d;
+    std::vector<int> blocks;
+    TableBuilder::Part<BQw> part;
+  };
+  std::vector<QGroup> qg;
+  for (BatchSys* sp : act) {
+    BatchSys& s = *sp;
+    const int k = s.k;
+    const int64_t n = s.n, nk = n * k;
+    const int kp = 32;
+    px.push_back(BPackX{s.b.p, (int64_t)k, n, s.ntiles, k, 0, 1, s.xh, s.xl, s.xm});
+    pxb.push_back((int)ceil_div(s.ntiles * 128, 256));
+    MGroup* g = nullptr;
+    for (auto& q : mg)
+      if (q.gm == s.gm && q.ns == s.ns && q.clamp == s.clamp) g = &q;
+    if (!g) {
+      mg.push_back(MGroup{s.gm, s.ns, s.clamp, {}, {}, {}});
+      g = &mg.back();
+    }
+    g->d.push_back(BMulti2{s.colpack, s.rowpack, s.ka, n, n, s.xh, s.xl, s.xm, k, 0, s.op->part.as<float>(),
+                           (int64_t)kp, s.jsplit, (float)(s.op->clamp_limit * kLog2eD)});
+    g->blocks.push_back((int)(ceil_div(n, 128) * s.jsplit));
+    rd.push_back(BReduce{s.op->part.as<float>(), n, (int64_t)kp, (int64_t)k, (int64_t)k, k, s.jsplit,
+                         s.op->lambda_reg, s.b.p, s.b.ap});
+    rdb.push_back((int)ceil_div(nk, 256));
+    cd3.push_back(BColdots{s.b.p, s.b.ap, s.b.p, s.b.p, s.b.ap, s.b.ap, n, s.b.rpb, k, s.b.nblk, s.b.part, s.b.st});
+    cdr.push_back(BColdots{s.b.r, s.b.r, nullptr, nullptr, nullptr, nullptr, n, s.b.rpb, k, s.b.nblk, s.b.part,
+                           s.b.st});
+    cdz.push_back(BColdots{s.b.r, s.b.z, nullptr, nullptr, nullptr, nullptr, n, s.b.rpb, k, s.b.nblk, s.b.part,
+                           s.b.st});
+    cdb.push_back(s.b.nblk);
+    sa.push_back(BScalar{s.b.part, s.b.nblk, k, s.b.rz, s.b.alpha, s.b.st});
+    sr.push_back(BScalar{s.b.part, s.b.nblk, k, s.b.rz, nullptr, s.b.st});
+    sb.push_back(BScalar{s.b.part, s.b.nblk, k, s.b.rz, s.b.beta, s.b.st});
+    ux.push_back(BUpdX64{s.X, s.b.r, s.b.p, s.b.ap, s.b.alpha, n, k, s.b.st});
+    uxb.push_back((int)ceil_div(nk, 256));
+    up.push_back(BUpdP{s.b.p, s.b.z, s.b.beta, n, k, s.b.st});
+    const BatchApplyView& v = s.av;
+    qt.push_back(BQtx{v.Q, n, (int64_t)k, v.rps, v.nr, k, v.cblk, s.b.r, v.part});
+    qtb.push_back(v.cblk * v.ns_eff);
+    const int wb = (int)ceil_div(v.width, 64);
+    r1.push_back(BRedParts{v.part, v.ns_eff, v.width, wb, 8, v.part2});
+    r1b.push_back(wb * 8);
+    r2.push_back(BRedParts{v.part2, 8, v.width, wb, 1, v.W1});
+    r2b.push_back(wb);
+    const int ggx = (int)ceil_div(k, 64), ggy = (int)ceil_div(v.nr, 64);
+    gm.push_back(BGemm{v.nr, k, v.nr, v.nr, k, k, 1.0, 0.0, v.S, v.W1, v.W2, ggx});
+    gmb.push_back(ggx * ggy);
+    QGroup* h = nullptr;
+    for (auto& q : qg)
+      if (q.nb == v.qw_nb && q.P == v.P) h = &q;
+    if (!h) {
+      qg.push_back(QGroup{v.qw_nb, v.P, {}, {}, {}});
+      h = &qg.back();
+    }
+    h->d.push_back(BQw{v.Q, v.W2, n, (int64_t)k, (int64_t)k, v.nr, v.P, k, 0, s.b.r, s.b.z, v.f});
+    h->blocks.push_back(v.qw_blocks);
+  }
+  const std::vector<int> ones(act.size(), 1);
+  auto tpx = tb.add(px, pxb);
+  for (auto& g : mg) g.part = tb.add(g.d, g.blocks);
+  auto trd = tb.add(rd, rdb);
+  auto tc3 = tb.add(cd3, cdb);
+  auto tcr = tb.add(cdr, cdb);
+  auto tcz = tb.add(cdz, cdb);
+  auto tsa = tb.add(sa, ones);
+  auto tsr = tb.add(sr, ones);
+  auto tsb = tb.add(sb, ones);
+  auto tux = tb.add(ux, uxb);
+  auto tup = tb.add(up, uxb);
+  auto tqt = tb.add(qt, qtb);
+  auto tr1 = tb.add(r1, r1b);
+  auto tr2 = tb.add(r2, r2b);
+  auto tgm = tb.add(gm, gmb);
+  for (auto& q : qg) q.part = tb.add(q.d, q.blocks);
+  const size_t bytes = round_up(tb.h.size(), 256);
+  if (bytes > bt.table_bytes) {
+    // the staging buffers only grow; the previous poll's copy has completed
+    LAKER_HIP_CHECK(hipStreamSynchronize(st));
+    if (bt.table_h) (void)hipHostFree(bt.table_h);
+    bt.table_h = nullptr;
+    LAKER_HIP_CHECK(hipHostMalloc(&bt.table_h, bytes * 2, hipHostMallocDefault));
+    LAKER_TRY(bt.table_d.ensure(bytes * 2));
+    bt.table_bytes = bytes * 2;
+  }
+  LAKER_HIP_CHECK(hipStreamSynchronize(st));  // the staging copy may be in use by the previous poll
+  memcpy(bt.table_h, tb.h.data(), tb.h.size());
+  LAKER_HIP_CHECK(hipMemcpyAsync(bt.table_d.ptr, bt.table_h, tb.h.size(), hipMemcpyHostToDevice, st));
+  const char* base = bt.table_d.as<char>();
+  const int nsys = (int)act.size();
+  const dim3 b256(256);
+  for (int q = 0; q < count; ++q) {
+    // A p
+    LAKER_TRY(batched_pack_x(tb.launch(tpx, base), st));
+    for (auto& g : mg) LAKER_TRY(batched_multi2(tb.launch(g.part, base), g.gm, g.ns, g.clamp, st));
+    LAKER_TRY(batched_reduce(tb.launch(trd, base), st));
+    // p.Ap, p.p, Ap.Ap ; alpha ; x, r update ; ||r||
+    {
+      const auto L = tb.launch(tc3, base);
+      hipLaunchKernelGGL((coldots_batched<3>), dim3((unsigned)L.blocks), b256, 0, st, L.desc, L.off, nsys);
+      LAKER_KCHECK("coldots_batched");
+    }
+    hipLaunchKernelGGL(alpha_batched, dim3((unsigned)nsys), b256, 0, st, tb.launch(tsa, base).desc, nsys);
+    LAKER_KCHECK("alpha_batched");
+    {
+      const auto L = tb.launch(tux, base);
+      hipLaunchKernelGGL(update_x64_batched, dim3((unsigned)L.blocks), b256, 0, st, L.desc, L.off, nsys);
+      LAKER_KCHECK("update_x64_batched");
+    }
+    {
+      const auto L = tb.launch(tcr, base);
+      hipLaunchKernelGGL((coldots_batched<1>), dim3((unsigned)L.blocks), b256, 0, st, L.desc, L.off, nsys);
+      LAKER_KCHECK("coldots_batched");
+    }
+    hipLaunchKernelGGL(resid_batched, dim3((unsigned)nsys), b256, 0, st, tb.launch(tsr, base).desc, nsys);
+    LAKER_KCHECK("resid_batched");
+    // z = P r
+    LAKER_TRY(batched_qtx(tb.launch(tqt, base), st));
+    LAKER_TRY(batched_reduce_parts(tb.launch(tr1, base), st));
+    LAKER_TRY(batched_reduce_parts(tb.launch(tr2, base), st));
+    LAKER_TRY(batched_gemm(tb.launch(tgm, base), st));
+    for (auto& q : qg) LAKER_TRY(batched_qw(tb.launch(q.part, base), q.nb, q.P, st));
+    // r.z ; beta ; p
+    {
+      const auto L = tb.launch(tcz, base);
+      hipLaunchKernelGGL((coldots_batched<1>), dim3((unsigned)L.blocks), b256, 0, st, L.desc, L.off, nsys);
+      LAKER_KCHECK("coldots_batched");
+    }
+    hipLaunchKernelGGL(beta_batched, dim3((unsigned)nsys), b256, 0, st, tb.launch(tsb, base).desc, nsys);
+    LAKER_KCHECK("beta_batched");
+    {
+      const auto L = tb.launch(tup, base);
+      hipLaunchKernelGGL(update_p_batched, dim3((unsigned)L.blocks), b256, 0, st, L.desc, L.off, nsys);
+      LAKER_KCHECK("update_p_batched");
+    }
+  }
+  return LAKER_OK;
+}
 
 }  // namespace laker
 
@@ -980,6 +1395,162 @@ int laker_solve_mixed(laker_operator_t oh, laker_precond_t ph, const double* B, 
   }
   result->residual_norm = std::min(rnorm, rbest);
   result->inner_iterations = it;
+  return LAKER_OK;
+}
+
+int laker_batch_create(int32_t max_systems, hipStream_t stream, laker_batch_t* out) {
+  LAKER_CHECK_ARG(out != nullptr && max_systems >= 1 && max_systems <= 4096, "bad batch arguments");
+  Batch* bt = new Batch();
+  bt->st = stream;
+  bt->max_sys = max_systems;
+  bt->slot.assign(max_systems, nullptr);
+  bt->hs.resize(max_systems);
+  const int rc = bt->states.ensure((size_t)max_systems * sizeof(PcgState));
+  if (rc != LAKER_OK) {
+    delete bt;
+    return rc;
+  }
+  *out = reinterpret_cast<laker_batch_t>(bt);
+  return LAKER_OK;
+}
+
+int laker_batch_destroy(laker_batch_t h) {
+  Batch* bt = reinterpret_cast<Batch*>(h);
+  if (bt) (void)hipStreamSynchronize(bt->st);
+  delete bt;
+  return LAKER_OK;
+}
+
+int laker_batch_add(laker_batch_t h, laker_operator_t oh, laker_precond_t ph, const double* B, int32_t k, double* X,
+                    double tol, int32_t max_outer, double inner_tol, int32_t inner_max_iter, int32_t* slot_out) {
+  LAKER_CHECK_ARG(h && oh && ph && B && X && slot_out, "bad batch_add arguments");
+  LAKER_CHECK_ARG(k >= 2 && k <= 32, "batched solves take 2 <= k <= 32 right-hand sides");
+  LAKER_CHECK_ARG(inner_tol > 0.0 && inner_tol < 1.0, "inner_tol must lie in (0, 1)");
+  Batch& bt = *reinterpret_cast<Batch*>(h);
+  Operator& op = *reinterpret_cast<Operator*>(oh);
+  Precond& pc = *reinterpret_cast<Precond*>(ph);
+  LAKER_CHECK_ARG(op.dtype == LAKER_F32, "batched solves need an FP32 operator");
+  LAKER_CHECK_ARG(precond_rows(pc) == op.n, "preconditioner and operator sizes differ");
+  int slot = -1;
+  for (int i = 0; i < bt.max_sys; ++i)
+    if (!bt.slot[i]) {
+      slot = i;
+      break;
+    }
+  LAKER_CHECK_ARG(slot >= 0, "batch is full");
+  std::unique_ptr<BatchSys> sp(new BatchSys());
+  BatchSys& s = *sp;
+  s.op = &op;
+  s.pc = &pc;
+  s.B = B;
+  s.X = X;
+  s.k = k;
+  s.n = op.n;
+  s.tol = tol;
+  s.max_outer = max_outer;
+  s.inner_tol = inner_tol;
+  s.inner_max_iter = inner_max_iter;
+  if (precond_batch_view(pc, k, &s.av) != LAKER_OK)
+    return fail(LAKER_EINVAL, "batched solves need an FP32 CCCP preconditioner with N_r % 4 == 0");
+  // the product of product_multi_f32 on the 3-way split (one 32-column chunk)
+  const int64_t n = op.n, nk = n * k;
+  const int ns16h = op.ka16h / 16, ns16 = op.ka16 / 16;
+  const bool f16s = op.f16s_ok && (ns16h == 2 || ns16h == 3);
+  LAKER_CHECK_ARG(f16s || ns16 == 2 || ns16 == 4 || ns16 == 6, "no batched product for this embedding width");
+  s.gm = f16s ? GRAM_F16S : LAKER_PREC_MIXED;
+  s.ns = f16s ? ns16h : ns16;
+  s.colpack = f16s ? op.self.col16h.ptr : op.self.col16.ptr;
+  s.rowpack = f16s ? op.self.row16h.ptr : op.self.row16.ptr;
+  s.ka = f16s ? op.ka16h : op.ka16;
+  s.clamp = op.clamp_for(LAKER_CLAMP_AUTO);
+  // column splits of a batched product: the batch fills the chip with its
+  // systems, so a system needs few splits (unlike a launch of its own, which
+  // takes up to 64); a function of n alone, so a system's partial sums do not
+  // depend on the batch
+  {
+    const int64_t row_blocks = ceil_div(n, 128), tiles = ceil_div(n, 32);
+    int js = (int)std::max<int64_t>(1, std::min<int64_t>(8, 256 / row_blocks));
+    while (js > 1 && tiles / js < 2) js /= 2;
+    s.jsplit = js;
+  }
+  LAKER_TRY(op.ensure_part((size_t)s.jsplit * n * 32 * sizeof(float)));
+  s.ntiles = ceil_div(n, 32);
+  const size_t part_bytes = (size_t)s.ntiles * 128 * 2 * sizeof(bf16x8);
+  LAKER_TRY(op.xpack.ensure(3 * part_bytes));
+  s.xh = op.xpack.as<bf16x8>();
+  s.xl = reinterpret_cast<bf16x8*>(op.xpack.as<char>() + part_bytes);
+  s.xm = reinterpret_cast<bf16x8*>(op.xpack.as<char>() + 2 * part_bytes);
+  LAKER_TRY(pcg_buffers<float>(op, k, s.b));
+  s.b.st = bt.states.as<PcgState>() + slot;
+  const size_t vb = round_up(nk * 8, 256);
+  LAKER_TRY(op.mixed_ws.ensure(3 * vb));
+  s.R = op.mixed_ws.as<double>();
+  s.Xbest = reinterpret_cast<double*>(op.mixed_ws.as<char>() + vb);
+  s.Rbest = reinterpret_cast<double*>(op.mixed_ws.as<char>() + 2 * vb);
+  s.res = laker_mixed_result{};
+  LAKER_TRY(frob_norm(B, nk, s.tmp, &s.bnorm, bt.st));
+  s.res.rhs_norm = s.bnorm;
+  s.res.residual_norm = s.bnorm;
+  LAKER_HIP_CHECK(hipMemsetAsync(X, 0, nk * 8, bt.st));
+  s.rnorm = s.rbest = s.bnorm;
+  s.Rin = B;
+  if (s.bnorm == 0.0) {
+    s.res.converged = 1;
+    s.res.residual_norm = 0.0;
+    s.finished = true;
+  } else {
+    LAKER_TRY(batch_step_begin(bt, s));
+  }
+  bt.slot[slot] = sp.release();
+  *slot_out = slot;
+  return LAKER_OK;
+}
+
+int laker_batch_step(laker_batch_t h, int32_t iterations, int32_t* finished, int32_t* nfinished, int32_t* nactive) {
+  LAKER_CHECK_ARG(h && finished && nfinished && iterations >= 1, "bad batch_step arguments");
+  Batch& bt = *reinterpret_cast<Batch*>(h);
+  std::vector<BatchSys*> act;
+  for (BatchSys* s : bt.slot)
+    if (s && s->running && !s->finished) act.push_back(s);
+  if (!act.empty()) {
+    LAKER_TRY(batch_iterate(bt, act, iterations));
+    LAKER_HIP_CHECK(hipMemcpyAsync(bt.hs.data(), bt.states.ptr, bt.max_sys * sizeof(PcgState), hipMemcpyDeviceToHost,
+                                   bt.st));
+    LAKER_HIP_CHECK(hipStreamSynchronize(bt.st));
+    for (int i = 0; i < bt.max_sys; ++i) {
+      BatchSys* s = bt.slot[i];
+      if (!s || !s->running || s->finished) continue;
+      const PcgState& hs = bt.hs[i];
+      s->it = hs.it;
+      if (hs.done != PCG_RUNNING) LAKER_TRY(batch_finish_step(bt, *s, hs));
+    }
+  }
+  int nf = 0, na = 0;
+  for (int i = 0; i < bt.max_sys; ++i) {
+    BatchSys* s = bt.slot[i];
+    if (!s) continue;
+    if (s->finished) finished[nf++] = i;
+    else ++na;
+  }
+  if (nf) LAKER_HIP_CHECK(hipStreamSynchronize(bt.st));  // finished solutions are complete in memory
+  *nfinished = nf;
+  if (nactive) *nactive = na;
+  return LAKER_OK;
+}
+
+int laker_batch_result(laker_batch_t h, int32_t slot, laker_mixed_result* result) {
+  LAKER_CHECK_ARG(h && result, "bad batch_result arguments");
+  Batch& bt = *reinterpret_cast<Batch*>(h);
+  LAKER_CHECK_ARG(slot >= 0 && slot < bt.max_sys && bt.slot[slot], "no system in this batch slot");
+  BatchSys* s = bt.slot[slot];
+  LAKER_CHECK_ARG(s->finished, "the system in this slot has not finished");
+  *result = s->res;
+  delete s;
+  bt.slot[slot] = nullptr;
+  if (result->status == LAKER_EBREAKDOWN)
+    return fail(LAKER_EBREAKDOWN,
+                "PCG breakdown: non-positive curvature detected (p^T A p <= 0). The operator may "
+                "be indefinite or the preconditioner may be unsuitable.");
   return LAKER_OK;
 }
 

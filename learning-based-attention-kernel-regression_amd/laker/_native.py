@@ -122,6 +122,13 @@ def _load():
         "laker_coldot": (c_int32, [c_void_p, c_void_p, c_int64, c_int32, c_int64, c_int32, c_void_p,
                                    c_void_p]),
         "laker_set_graphs": (c_int32, [c_int32]),
+        "laker_batch_create": (c_int32, [c_int32, c_void_p, ctypes.POINTER(c_void_p)]),
+        "laker_batch_destroy": (c_int32, [c_void_p]),
+        "laker_batch_add": (c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_double, c_int32,
+                                      c_double, c_int32, ctypes.POINTER(c_int32)]),
+        "laker_batch_step": (c_int32, [c_void_p, c_int32, ctypes.POINTER(c_int32), ctypes.POINTER(c_int32),
+                                       ctypes.POINTER(c_int32)]),
+        "laker_batch_result": (c_int32, [c_void_p, c_int32, ctypes.POINTER(MixedResult)]),
         "laker_precond_create_jacobi": (c_int32, [c_void_p, c_int64, c_int32, c_void_p, ctypes.POINTER(c_void_p)]),
         "laker_precond_from_factors": (c_int32, [c_void_p, c_int64, c_int32, c_int32, c_double, c_double, c_void_p,
                                                  c_void_p, c_void_p, c_void_p, ctypes.POINTER(c_void_p)]),
@@ -146,7 +153,8 @@ EXPORTED = [
     "laker_precond_build_from_responses", "laker_precond_copy_out", "laker_precond_destroy",
     "laker_precond_info", "laker_precond_apply", "laker_pcg_solve", "laker_solve_mixed",
     "laker_set_graphs", "laker_coldot", "laker_precond_create_jacobi", "laker_precond_from_factors",
-    "laker_orthonormalize", "laker_randn",
+    "laker_orthonormalize", "laker_randn", "laker_batch_create", "laker_batch_destroy", "laker_batch_add",
+    "laker_batch_step", "laker_batch_result",
 ]
 
 

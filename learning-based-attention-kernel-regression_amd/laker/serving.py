@@ -1,17 +1,27 @@
 """Continuous-batching solve server for streams of LAKER systems on one GPU.
 
 Requests (locations, targets with k right-hand sides, embedding_dim) arrive
-over time.  The server keeps ``slots`` solve lanes, each a host worker bound
+over time.  The server keeps ``slots`` build lanes, each a host worker bound
 to its own HIP stream; a dispatcher admits a waiting request as soon as a lane
-is free *and* the request's HBM footprint fits the remaining budget, so
-requests that arrive while others are on the GPU start while those are still
-running and their kernels overlap on the device.  Each admitted request runs
-the full device path -- embedding, operator packing, CCCP build, mixed- or
-FP32-precision PCG -- entirely stream-ordered on its lane; E, the factors and
-the Krylov state stay resident until the request completes.
+is free *and* the request's HBM footprint fits the remaining budget.  A lane
+computes the request's embedding, packs its operator and builds its CCCP
+preconditioner.  Then:
 
-Results are deterministic per request (fixed-order reductions, probes seeded
-by the request id), so a request's solution is bit-identical whether it is
+* small systems (n <= ``batch_max_n``, 2..32 right-hand sides, mixed
+  precision) are handed to the batch: one device batch (``MixedBatch``, one
+  HIP stream) runs the inner PCG iterations of every system in it as shared
+  kernel launches -- a launch per kernel of the iteration for all of them --
+  and takes each system's FP64 refinement steps between polls; systems join
+  as soon as their build is done and a batch slot is free, and leave when
+  they have converged, so the batch is continuously refilled;
+* other systems solve on their lane (mixed- or FP32-precision PCG, HIP-graph
+  replay of the iterations), overlapping with the batch on the device.
+
+E, the factors and the Krylov state stay resident until the request
+completes; its HBM footprint stays reserved until then.  Results are
+deterministic per request (fixed-order reductions, probes seeded by the
+request id, and a batched system's iterates do not depend on the other
+systems in the batch), so a request's solution is bit-identical whether it is
 solved alone or in the mix.  Latency = completion - arrival (queueing
 included); the server reports solves/sec and p50/p99.
 
@@ -21,6 +31,7 @@ No reference counterpart: the reference solves one system per call
 
 from __future__ import annotations
 
+import collections
 import dataclasses
 import math
 import threading
@@ -62,6 +73,7 @@ class SolveResult:
     relres: float = float("nan")
     slot: int = -1
     error: Optional[str] = None
+    batched: bool = False
 
 
 def footprint_bytes(n: int, k: int, d: int, nr: Optional[int] = None) -> int:
@@ -110,18 +122,23 @@ class SolveServer:
     """Continuous-batching server (see module docstring).
 
     Args:
-        slots: concurrent solve lanes (HIP streams).
+        slots: build lanes (HIP streams) -- and solve lanes for systems
+            outside the batch.
         hbm_budget_bytes: admission cap on the summed request footprints
             (default: 70% of the GPU's memory).
         precision: ``"mixed"`` or ``"fp32"``.
         tol / inner_tol: PCG tolerances.
+        batch_max_n: largest n solved in the device batch (0 disables the
+            batch; mixed precision only).
+        batch_systems: systems the batch solves at once.
         solve_fn: injectable per-request work ``(req, slot) -> SolveResult``
-            (host-side tests of the scheduler use a stand-in).
+            (host-side tests of the scheduler use a stand-in; no batch then).
     """
 
     def __init__(self, slots: int = 4, hbm_budget_bytes: Optional[int] = None, precision: str = "mixed",
                  tol: float = 1e-6, inner_tol: float = 5e-2, gamma: float = 0.1,
-                 solve_fn: Optional[Callable] = None, footprint_fn: Callable = footprint_bytes):
+                 solve_fn: Optional[Callable] = None, footprint_fn: Callable = footprint_bytes,
+                 batch_max_n: int = 8192, batch_systems: int = 64):
         self.slots = int(slots)
         self.precision = precision
         self.tol = tol
@@ -133,10 +150,14 @@ class SolveServer:
             hbm_budget_bytes = int(0.7 * torch.cuda.get_device_properties(0).total_memory) \
                 if torch.cuda.is_available() else 1 << 40
         self.budget = int(hbm_budget_bytes)
+        self.batch_max_n = int(batch_max_n) if precision == "mixed" and solve_fn is None else 0
+        self.batch_systems = int(batch_systems)
         self._emb = {}
         self._streams = None
+        self._batch = None
         self.max_inflight = 0
         self.max_bytes_inflight = 0
+        self.max_batched = 0
 
     # ------------------------------------------------------------------
     def _embedding(self, d: int):
@@ -147,12 +168,14 @@ class SolveServer:
             self._emb[d] = PositionEmbedding(2, d, device=compute_device())
         return self._emb[d]
 
-    def solve_one(self, req: SolveRequest, slot: int) -> SolveResult:
-        """Device path of one request on the current stream."""
+    def batchable(self, req: SolveRequest) -> bool:
+        return self.batch_max_n > 0 and req.n <= self.batch_max_n and 2 <= req.k <= 32
+
+    def build_one(self, req: SolveRequest):
+        """Embedding, operator and CCCP preconditioner of a request, on the current stream."""
         from .backend import compute_device
         from .kernels import AttentionKernelOperator
         from .preconditioner import CCCPPreconditioner
-        from .solvers import PreconditionedConjugateGradient
 
         dev = compute_device()
         with torch.no_grad():
@@ -163,6 +186,13 @@ class SolveServer:
         pre = CCCPPreconditioner(gamma=self.gamma, verbose=False, probe_source="device",
                                  probe_precision="mixed" if self.precision == "mixed" else "fp32")
         pre.build(op.matvec, req.n, seed=req.rid)
+        return op, pre, y
+
+    def solve_one(self, req: SolveRequest, slot: int) -> SolveResult:
+        """Device path of one request on the current stream (outside the batch)."""
+        from .solvers import PreconditionedConjugateGradient
+
+        op, pre, y = self.build_one(req)
         pcg = PreconditionedConjugateGradient(tol=self.tol, max_iter=1000, verbose=False,
                                               precision=self.precision, inner_tol=self.inner_tol)
         a = pcg.solve(op.matvec, pre.apply, y)
@@ -186,38 +216,113 @@ class SolveServer:
         use_gpu = self._solve_fn is None
         if use_gpu and self._streams is None:
             self._streams = [torch.cuda.Stream() for _ in range(self.slots)]
+        use_batch = use_gpu and self.batch_max_n > 0 and any(self.batchable(r) for r in reqs)
+        if use_batch and self._batch is None:
+            from .batching import MixedBatch
+
+            self._batch = MixedBatch(max_systems=self.batch_systems, tol=self.tol, inner_tol=self.inner_tol)
         lock = threading.Condition()
         free_slots = list(range(self.slots))
         inflight_bytes = [0]
         inflight = [0]
         results: List[SolveResult] = []
+        handoff = collections.deque()   # built systems waiting for a batch slot
+        remaining = [len(reqs)]
         t0 = time.perf_counter()
 
-        def worker(req: SolveRequest, slot: int, t_admit: float):
-            try:
-                if use_gpu:
-                    with torch.cuda.stream(self._streams[slot]):
-                        res = self.solve_one(req, slot)
-                        torch.cuda.current_stream().synchronize()
-                else:
-                    res = self._solve_fn(req, slot)
-            except Exception as exc:  # reported per request, the server keeps going
-                res = SolveResult(rid=req.rid, alpha=None, n=req.n, k=req.k, latency_s=0, queue_s=0,
-                                  service_s=0, slot=slot, error=repr(exc))
+        def finish(res: SolveResult, req: SolveRequest, t_admit: float, lane: Optional[int] = None):
             t_done = time.perf_counter() - t0
             arrival = req.arrival if realtime else 0.0
             res.latency_s = t_done - arrival
             res.queue_s = t_admit - arrival
             res.service_s = t_done - t_admit
-            res.slot = slot
             with lock:
+                if lane is not None:
+                    free_slots.append(lane)
                 results.append(res)
-                free_slots.append(slot)
                 inflight_bytes[0] -= foot[req.rid]
                 inflight[0] -= 1
+                remaining[0] -= 1
                 lock.notify_all()
 
+        def error_result(req, slot, exc):
+            return SolveResult(rid=req.rid, alpha=None, n=req.n, k=req.k, latency_s=0, queue_s=0, service_s=0,
+                               slot=slot, error=repr(exc))
+
+        def worker(req: SolveRequest, slot: int, t_admit: float):
+            res = None
+            try:
+                if use_gpu:
+                    with torch.cuda.stream(self._streams[slot]):
+                        if use_batch and self.batchable(req):
+                            built = self.build_one(req)
+                            torch.cuda.current_stream().synchronize()
+                            with lock:
+                                handoff.append((req, slot, t_admit, built))
+                                free_slots.append(slot)
+                                lock.notify_all()
+                            return
+                        res = self.solve_one(req, slot)
+                        torch.cuda.current_stream().synchronize()
+                else:
+                    res = self._solve_fn(req, slot)
+            except Exception as exc:  # reported per request, the server keeps going
+                res = error_result(req, slot, exc)
+            res.slot = slot
+            finish(res, req, t_admit, lane=slot)
+
+        def batch_loop():
+            batch = self._batch
+            waiting = {}
+            while True:
+                with lock:
+                    while not handoff and not batch.entries and remaining[0] > 0:
+                        lock.wait(timeout=0.05)
+                    if remaining[0] == 0 and not batch.entries and not handoff:
+                        return
+                    joins = []
+                    room = batch.free
+                    while handoff and room > 0:
+                        joins.append(handoff.popleft())
+                        room -= 1
+                for req, slot, t_admit, (op, pre, y) in joins:
+                    try:
+                        e = batch.add(op, pre, y if y.dim() == 2 else y.unsqueeze(1), key=req.rid)
+                        waiting[req.rid] = (req, slot, t_admit, op, pre, e)
+                    except Exception as exc:
+                        op.close()
+                        pre.close()
+                        finish(error_result(req, slot, exc), req, t_admit)
+                self.max_batched = max(self.max_batched, len(batch.entries))
+                if not batch.entries:
+                    continue
+                try:
+                    done = batch.step()
+                except Exception as exc:  # a failed poll ends every system in the batch
+                    done = list(batch.entries.values())
+                    batch.entries.clear()
+                    for e in done:
+                        e.error = repr(exc)
+                for e in done:
+                    req, slot, t_admit, op, pre, _ = waiting.pop(e.key)
+                    if e.error is not None:
+                        res = error_result(req, slot, e.error)
+                    else:
+                        r = e.result
+                        alpha = e.x if req.y.dim() == 2 else e.x.squeeze(1)
+                        res = SolveResult(rid=req.rid, alpha=alpha, n=req.n, k=req.k, latency_s=0.0, queue_s=0.0,
+                                          service_s=0.0, iterations=r.inner_iterations, outer=r.outer_iterations,
+                                          relres=r.residual_norm / r.rhs_norm if r.rhs_norm else 0.0, slot=slot,
+                                          batched=True)
+                    op.close()
+                    pre.close()
+                    finish(res, req, t_admit)
+
         threads = []
+        if use_batch:
+            bth = threading.Thread(target=batch_loop, daemon=True)
+            bth.start()
+            threads.append(bth)
         i = 0
         with lock:
             while i < len(reqs):
