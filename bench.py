@@ -127,7 +127,11 @@ def parse():
                     help="only launch the k=1 matvec this many times per precision (profiling)")
     ap.add_argument("--serving-requests", type=int, default=200,
                     help="requests in the serving sample: ids 0..R-1 of the stream (0 disables it)")
-    ap.add_argument("--serving-slots", type=int, default=4)
+    ap.add_argument("--serving-slots", type=int, default=8,
+                    help="build lanes of the serving sample (and solve lanes of the systems outside the batch)")
+    ap.add_argument("--serving-small", type=int, default=128,
+                    help="saturating sample of small requests (n <= 8192, 2..32 RHS) solved batched and per lane "
+                         "(0 disables it)")
     ap.add_argument("--serving-rate", type=float, default=4.0,
                     help="Poisson arrival rate (requests/s, seed 0) of the serving sample, honoured against "
                          "the wall clock; 0 makes every request available at t=0 (saturating)")
@@ -472,7 +476,32 @@ def main():
         res_q = seq.run(reqs[:args.serving_seq or len(reqs)], realtime=False)
         sm, sq = SolveServer.summary(res_s, srv.elapsed_s), SolveServer.summary(res_q, seq.elapsed_s)
         worst = max((r.relres for r in res_s if r.error is None), default=float("nan"))
-        serving = {"requests": len(reqs), "slots": args.serving_slots,
+        small = None
+        if args.serving_small > 0:
+            # the first requests of the stream with n <= 8192 and 2..32 right-hand sides, all at t=0
+            sreqs, rid = [], 0
+            while len(sreqs) < args.serving_small:
+                q = make_request(rid)
+                if q.n <= 8192 and q.k >= 2:
+                    sreqs.append(q)
+                rid += 1
+            small = {"requests": len(sreqs), "n_range": [min(q.n for q in sreqs), max(q.n for q in sreqs)],
+                     "rhs_total": sum(q.k for q in sreqs), "arrivals": "all at t=0 (saturating)",
+                     "slots": args.serving_slots}
+            for name, bmax in (("batched", 8192), ("per_lane", 0)):
+                sv = SolveServer(slots=args.serving_slots, precision="mixed", tol=args.tol, inner_tol=args.inner_tol,
+                                 batch_max_n=bmax)
+                sv.run(sreqs[:4], realtime=False)  # warm-up
+                rs = sv.run(sreqs, realtime=False)
+                small[name] = SolveServer.summary(rs, sv.elapsed_s)
+                small[name]["max_fp64_relres"] = max((r.relres for r in rs if r.error is None), default=float("nan"))
+                if bmax:
+                    small[name]["max_systems_in_batch"] = sv.max_batched
+            small["batched_vs_per_lane"] = small["batched"]["solves_per_sec"] / small["per_lane"]["solves_per_sec"]
+            print(f"[bench] small requests: batched {small['batched']['solves_per_sec']:.1f} solves/s p99 "
+                  f"{small['batched']['p99_ms']:.0f} ms; per lane {small['per_lane']['solves_per_sec']:.1f} solves/s "
+                  f"p99 {small['per_lane']['p99_ms']:.0f} ms", file=sys.stderr, flush=True)
+        serving = {"requests": len(reqs), "slots": args.serving_slots, "small_requests": small,
                    "arrivals": (f"Poisson, {args.serving_rate}/s, seed 0, wall clock" if realtime
                                 else "all at t=0 (saturating)"),
                    "n_range": [min(r.n for r in reqs), max(r.n for r in reqs)],
