@@ -9,33 +9,27 @@ namespace laker {
 
 // ---------------------------------------------------------------------------
 // Symmetric k = 1 product of the square operator.  Workgroup (b, seg) owns
-// rows [SYM_RB b, SYM_RB (b + 1)) (4 waves x SYM_ITW row tiles) and the column
-// tiles of segment seg, [SYM_SEG seg, SYM_SEG (seg + 1)), at or right of tile
-// SYM_DT b: only tiles at or right of the diagonal SYM_RB x SYM_RB block are
-// visited, so every exponential is evaluated once for the pair (i, j), (j, i).
-// Each exp tile P feeds the row sums y_I += P x_J (in-lane, as in the
-// non-symmetric kernel) and, right of the diagonal block, the column sums
-// y_J += P^T x_I: the products of the wave's row tiles are summed in-lane,
-// then a 32-lane transpose-reduce (one permlane16 swap and four DPP levels)
-// runs once per column tile for all of them, a fixed-order sum over the 4
-// waves in LDS follows, and one FP64 partial is stored per (row block,
-// column).  Row partials: part_r[seg][i]; column partials:
+// rows [SYM_RB b, SYM_RB (b + 1)) (SYM_WAVES waves x SYM_ITW row tiles of 32)
+// and the column tiles of segment seg, [SYM_SEG seg, SYM_SEG (seg + 1)), at
+// or right of tile SYM_DT b: only tiles at or right of the diagonal
+// SYM_RB x SYM_RB block are visited, so every exponential is evaluated once
+// for the pair (i, j), (j, i).  Each exp tile P feeds the row sums
+// y_I += P x_J and, right of the diagonal block, the column sums
+// y_J += P^T x_I.  The Gram tile is formed as D = B_row A_col^T, so a lane
+// holds one column (li) and 16 rows (jj_of(v, h)) of each 32 x 32 tile:
+// * column sums are in-lane FMAs over the wave's rows, one swap between the
+//   half-waves (same column, other rows), a fixed-order sum over the waves
+//   in LDS and one FP32 partial per (row block, column);
+// * row sums accumulate per (row, lane column) in FP32 registers over the
+//   whole segment (one FMA per entry, 32 independent chains per lane), and
+//   one 32-lane transpose-reduce per row tile at the end of the segment
+//   gives one FP32 partial per (segment, row).
+// Per pair of entries that is one v_exp_f32, one row FMA and one column
+// FMA, the VALU-issue floor of the scheme, plus a few instructions per
+// 32 x 32 tile.  Row partials: part_r[seg][i]; column partials:
 // part_c[coff(b) + j - SYM_RB (b + 1)] for j >= SYM_RB (b + 1),
 // coff(b) = b n - (SYM_RB / 2) b (b + 1).
-// The kernel is bound by VALU issue (v_exp_f32 takes two FMA slots): per
-// pair of entries one exp, one row FMA and one column FMA; the transpose-
-// reduce (about 40 instructions per column tile) is what SYM_ITW amortises.
 // ---------------------------------------------------------------------------
-
-#ifndef LAKER_SYM_RACC
-#define LAKER_SYM_RACC 0
-#endif
-#ifndef LAKER_SYM_ALL
-#define LAKER_SYM_ALL 0
-#endif
-#ifndef LAKER_SYM_UNROLLTT
-#define LAKER_SYM_UNROLLTT 0
-#endif
 
 template <int CTRL>
 __device__ inline float dpp_f(float v) {
@@ -76,37 +70,42 @@ __device__ inline float transpose_reduce16(const float (&q)[16], int lane) {
 }
 
 
-// Waves per SIMD the register allocation must allow: the FP16-split
-// instance with at most two Gram steps fits 168 VGPRs without spills and runs
-// three waves per SIMD instead of two; the others keep the compiler's choice.
+using SymPart = float;  // partial sums of the symmetric product
+// Waves per SIMD the register allocation must allow: 8-wave workgroups
+// occupy two waves per SIMD each, so a second workgroup per CU needs the
+// kernel within 128 VGPRs (four waves per SIMD; measured 0.77 ms at
+// n = 100,000 against 0.94 ms at two); the instances whose operands do not
+// fit keep the compiler's choice.
 template <int MODE, int NS>
-constexpr int sym1_min_waves() { return MODE == GRAM_F16S && NS <= 2 ? 3 : 1; }
+constexpr int sym1_min_waves() {
+  return (MODE == GRAM_F16S && NS <= 2) || (MODE == LAKER_PREC_FP32 && NS <= 5) ? 4 : 1;
+}
 
 template <int MODE, int NS, bool CLAMP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sym1_min_waves<MODE, NS>()))) void mv_sym1_kernel(
-    const void* __restrict__ colpack, const void* __restrict__ rowpack, int64_t n,
-    const float* __restrict__ X, int64_t ldx, const int2* __restrict__ map, double* __restrict__ part_r,
-    double* __restrict__ part_c, float clampv, const int* __restrict__ skip) {
+__global__ __launch_bounds__(64 * SYM_WAVES) __attribute__((amdgpu_waves_per_eu(sym1_min_waves<MODE, NS>()))) void
+mv_sym1_kernel(const void* __restrict__ colpack, const void* __restrict__ rowpack, int64_t n,
+               const float* __restrict__ X, int64_t ldx, const int2* __restrict__ map, float* __restrict__ part_r,
+               float* __restrict__ part_c, float clampv, const int* __restrict__ skip) {
   using G = GramOps<MODE>;
-  constexpr int ITW = SYM_ITW;
+  constexpr int ITW = SYM_ITW, NW = SYM_WAVES, NT = 64 * SYM_WAVES, JT = SYM_JT;
   if (skip && *skip) return;  // uniform across the grid
   extern __shared__ __align__(16) char sym_smem[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, h = lane >> 5, li = lane & 31;
   const int2 bs = map[blockIdx.x];
   const int64_t b = bs.x;
-  const int64_t I0 = b * SYM_RB + w * (32 * ITW);
+  const int64_t R0 = b * SYM_RB;
+  const int64_t I0 = R0 + w * (32 * ITW);
   const bool active = I0 < n;
-  const int64_t wrem = (n - b * SYM_RB + 32 * ITW - 1) / (32 * ITW);
-  const int nact = wrem < 4 ? (int)wrem : 4;
-  constexpr int gb = MODE == LAKER_PREC_FP32 ? 8 * NS : 32 * NS;
+  constexpr int gb = MODE == LAKER_PREC_FP32 ? 8 * NS : 32 * NS;  // bytes of one packed row
   constexpr int lb = MODE == LAKER_PREC_FP32 ? ((gb / 4) % 2 == 0 ? gb + 4 : gb) : gb + 16;
-  constexpr int chunk = V1_JT * 32 * (lb + 4);
+  constexpr int chunk = JT * 32 * (lb + 4);
   char* const buf0 = sym_smem;
   char* const buf1 = sym_smem + chunk;
-  float* const colbuf = reinterpret_cast<float*>(sym_smem + 2 * chunk);  // [2][4][V1_JT*32]
+  float* const colbuf = reinterpret_cast<float*>(sym_smem + 2 * chunk);  // [2][NW][JT*32]
+  float* const xrow = colbuf + 2 * NW * JT * 32;                         // x of the block's rows
 
+  for (int t = threadIdx.x; t < SYM_RB; t += NT) xrow[t] = R0 + t < n ? X[(R0 + t) * ldx] : 0.f;
   typename G::template Operand<NS> B[ITW];
-  float xr[ITW];
   if (active) {
     // packed rows exist up to round_up(n, 64): a row tile starting at or
     // beyond n (the wave's last tiles in the last row block) stays zero
@@ -116,8 +115,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sym1_min_wa
         G::load(B[it], rowpack, I0 + it * 32 + li, NS * (MODE == LAKER_PREC_FP32 ? 2 : 16), h);
       else
         B[it] = typename G::template Operand<NS>{};
-      const int64_t i = I0 + it * 32 + li;
-      xr[it] = i < n ? X[i * ldx] : 0.f;
     }
   }
   const int64_t ntiles = (n + 31) / 32;
@@ -127,16 +124,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sym1_min_wa
   const int64_t t0 = (SYM_DT * b > (int64_t)bs.y * SYM_SEG) ? SYM_DT * b : (int64_t)bs.y * SYM_SEG;
   const int64_t t1 = ((int64_t)bs.y * SYM_SEG + SYM_SEG < ntiles) ? (int64_t)bs.y * SYM_SEG + SYM_SEG : ntiles;
   const char* gsrc = reinterpret_cast<const char*>(colpack);
-  constexpr int VPT = MODE == LAKER_PREC_FP32 ? (8 * NS + 31) / 32 : NS;
+  constexpr int NVEC = JT * 32 * gb / 16;  // 16-byte vectors of a full chunk
+  constexpr int VPT = (NVEC + NT - 1) / NT;
   uint4 pre[VPT];
   float prex = 0.f;
   auto fetch = [&](int64_t t) {
-    const int nt = (int)((t1 - t) < V1_JT ? (t1 - t) : V1_JT);
+    const int nt = (int)((t1 - t) < JT ? (t1 - t) : JT);
     const int nvec = nt * 2 * gb;
     const uint4* src = reinterpret_cast<const uint4*>(gsrc + t * 32 * gb);
 #pragma unroll
     for (int u = 0; u < VPT; ++u) {
-      const int vi = u * 256 + threadIdx.x;
+      const int vi = u * NT + threadIdx.x;
       pre[u] = (vi < nvec) ? src[vi] : uint4{0u, 0u, 0u, 0u};
     }
     const int64_t j = t * 32 + threadIdx.x;
@@ -147,51 +145,46 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sym1_min_wa
       constexpr int wpr = gb / 4, lwpr = lb / 4;
 #pragma unroll
       for (int u = 0; u < VPT; ++u) {
-        const int wi = (u * 256 + threadIdx.x) * 4;
+        const int wi = (u * NT + threadIdx.x) * 4;
         const uint32_t vals[4] = {pre[u].x, pre[u].y, pre[u].z, pre[u].w};
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int word = wi + q;
-          if (word < V1_JT * 32 * wpr)
-            reinterpret_cast<uint32_t*>(dst)[(word / wpr) * lwpr + word % wpr] = vals[q];
+          if (word < JT * 32 * wpr) reinterpret_cast<uint32_t*>(dst)[(word / wpr) * lwpr + word % wpr] = vals[q];
         }
       }
     } else {
       constexpr int vpr = gb / 16;
 #pragma unroll
       for (int u = 0; u < VPT; ++u) {
-        const int vi = u * 256 + threadIdx.x;
-        if (vi < V1_JT * 32 * vpr)
-          *reinterpret_cast<uint4*>(dst + (vi / vpr) * lb + (vi % vpr) * 16) = pre[u];
+        const int vi = u * NT + threadIdx.x;
+        if (vi < NVEC) *reinterpret_cast<uint4*>(dst + (vi / vpr) * lb + (vi % vpr) * 16) = pre[u];
       }
     }
-    if (threadIdx.x < V1_JT * 32) reinterpret_cast<float*>(dst + V1_JT * 32 * lb)[threadIdx.x] = prex;
+    if (threadIdx.x < JT * 32) reinterpret_cast<float*>(dst + JT * 32 * lb)[threadIdx.x] = prex;
   };
-  // lane -> column of its transpose-reduce result (see transpose_reduce16)
-  const int vq = ((lane >> 1) & 1) + 2 * ((lane >> 2) & 1) + 4 * ((lane >> 3) & 1) + 8 * ((lane >> 4) & 1);
-  const int jq = jj_of(vq, h);
 
-  double acc64[ITW];
+  float racc[ITW][16];
 #pragma unroll
-  for (int it = 0; it < ITW; ++it) acc64[it] = 0.0;
+  for (int it = 0; it < ITW; ++it)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) racc[it][v] = 0.f;
   fetch(t0);
   stash(buf0);
   __syncthreads();
   bool odd = false;
-  for (int64_t tc = t0; tc < t1; tc += V1_JT) {
-    const bool more = tc + V1_JT < t1;
-    if (more) fetch(tc + V1_JT);
-    const int nt = (int)((t1 - tc) < V1_JT ? (t1 - tc) : V1_JT);
-    float* cb = colbuf + (odd ? 4 * V1_JT * 32 : 0);
+  for (int64_t tc = t0; tc < t1; tc += JT) {
+    const bool more = tc + JT < t1;
+    if (more) fetch(tc + JT);
+    const int nt = (int)((t1 - tc) < JT ? (t1 - tc) : JT);
+    float* cb = colbuf + (odd ? NW * JT * 32 : 0);
     if (active) {
       const char* cur = odd ? buf1 : buf0;
-      const float* xs = reinterpret_cast<const float*>(cur + V1_JT * 32 * lb);
-#if LAKER_SYM_RACC
-      // row sums of the chunk in FP32 (two chains per row tile), FP64 per chunk
-      float racc[ITW][2];
-#pragma unroll
-      for (int it = 0; it < ITW; ++it) racc[it][0] = racc[it][1] = 0.f;
-#endif
+      const float* xs = reinterpret_cast<const float*>(cur + JT * 32 * lb);
+      // Every tile takes row and column sums: the column sums of tiles inside
+      // the diagonal block (t < tdiag) and of the zero-filled tiles of a short
+      // last chunk are never read.  Zero-filled tiles contribute exp(0) * 0
+      // to the row sums.
       auto tile = [&](int tt) __attribute__((always_inline)) {
         typename G::template Operand<NS> A;
         const char* rowp = cur + (tt * 32 + li) * lb;
@@ -204,100 +197,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sym1_min_wa
           for (int q = 0; q < NS; ++q)
             A.v[q] = __builtin_bit_cast(typename G::vec, *reinterpret_cast<const shortx8*>(rowp + q * 32 + h * 16));
         }
-        float xv[16];
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-          const floatx4 f = *reinterpret_cast<const floatx4*>(xs + tt * 32 + 8 * g + 4 * h);
-#pragma unroll
-          for (int q = 0; q < 4; ++q) xv[4 * g + q] = f[q];
-        }
-        float q[16];
-#if LAKER_SYM_ALL
-        // all row tiles' Gram tiles first: later MFMAs run under earlier exps
-        floatx16 D[ITW];
-#pragma unroll
-        for (int it = 0; it < ITW; ++it) D[it] = G::template gram_exact<NS>(A, B[it]);
+        const float xc = xs[tt * 32 + li];
+        float q0 = 0.f, q1 = 0.f;
 #pragma unroll
         for (int it = 0; it < ITW; ++it) {
+          const floatx16 D = G::template gram_exact<NS>(B[it], A);
+          float xv[16];
 #pragma unroll
-          for (int v = 0; v < 16; ++v) D[it][v] = kexp<CLAMP, false>(D[it][v], clampv);
-#if LAKER_SYM_RACC
-          float a0 = racc[it][0], a1 = racc[it][1];
+          for (int g = 0; g < 4; ++g) {
+            const floatx4 f = *reinterpret_cast<const floatx4*>(xrow + w * (32 * ITW) + it * 32 + 8 * g + 4 * h);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) xv[4 * g + q] = f[q];
+          }
 #pragma unroll
           for (int v = 0; v < 16; v += 2) {
-            a0 = __builtin_fmaf(D[it][v], xv[v], a0);
-            a1 = __builtin_fmaf(D[it][v + 1], xv[v + 1], a1);
+            const float e0 = kexp<CLAMP, false>(D[v], clampv), e1 = kexp<CLAMP, false>(D[v + 1], clampv);
+            racc[it][v] = __builtin_fmaf(e0, xc, racc[it][v]);
+            racc[it][v + 1] = __builtin_fmaf(e1, xc, racc[it][v + 1]);
+            q0 = __builtin_fmaf(e0, xv[v], q0);
+            q1 = __builtin_fmaf(e1, xv[v + 1], q1);
           }
-          racc[it][0] = a0;
-          racc[it][1] = a1;
-#else
-          float a0 = D[it][0] * xv[0], a1 = D[it][1] * xv[1];
-#pragma unroll
-          for (int v = 2; v < 16; v += 2) {
-            a0 = __builtin_fmaf(D[it][v], xv[v], a0);
-            a1 = __builtin_fmaf(D[it][v + 1], xv[v + 1], a1);
-          }
-          acc64[it] += (double)(a0 + a1);
-#endif
-#pragma unroll
-          for (int v = 0; v < 16; ++v) q[v] = it == 0 ? D[0][v] * xr[0] : __builtin_fmaf(D[it][v], xr[it], q[v]);
         }
-#else
-        // row tiles in pairs: two Gram tiles live at a time
-#pragma unroll
-        for (int p = 0; p < ITW / 2; ++p) {
-          floatx16 D[2];
-#pragma unroll
-          for (int u = 0; u < 2; ++u) D[u] = G::template gram_exact<NS>(A, B[2 * p + u]);
-#pragma unroll
-          for (int u = 0; u < 2; ++u) {
-#pragma unroll
-            for (int v = 0; v < 16; ++v) D[u][v] = kexp<CLAMP, false>(D[u][v], clampv);
-#if LAKER_SYM_RACC
-            float a0 = racc[2 * p + u][0], a1 = racc[2 * p + u][1];
-#pragma unroll
-            for (int v = 0; v < 16; v += 2) {
-              a0 = __builtin_fmaf(D[u][v], xv[v], a0);
-              a1 = __builtin_fmaf(D[u][v + 1], xv[v + 1], a1);
-            }
-            racc[2 * p + u][0] = a0;
-            racc[2 * p + u][1] = a1;
-#else
-            float a0 = D[u][0] * xv[0], a1 = D[u][1] * xv[1];
-#pragma unroll
-            for (int v = 2; v < 16; v += 2) {
-              a0 = __builtin_fmaf(D[u][v], xv[v], a0);
-              a1 = __builtin_fmaf(D[u][v + 1], xv[v + 1], a1);
-            }
-            acc64[2 * p + u] += (double)(a0 + a1);
-#endif
-          }
-          // column-side products of the row tiles, summed in-lane
-#pragma unroll
-          for (int v = 0; v < 16; ++v)
-            q[v] = __builtin_fmaf(D[1][v], xr[2 * p + 1],
-                                  p == 0 ? D[0][v] * xr[0] : __builtin_fmaf(D[0][v], xr[2 * p], q[v]));
-        }
-#endif
-        if (tc + tt >= tdiag) {  // uniform across the block
-          const float cs = transpose_reduce16(q, lane);
-          if ((lane & 1) == 0) cb[w * (V1_JT * 32) + tt * 32 + jq] = cs;
-        }
+        const float cs = q0 + q1;
+        // lanes l and l ^ 32 hold the same column (rows 4h + ...)
+        const auto pr = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(unsigned, cs),
+                                                         __builtin_bit_cast(unsigned, cs), false, false);
+        const float tot = __builtin_bit_cast(float, (unsigned)pr[0]) + __builtin_bit_cast(float, (unsigned)pr[1]);
+        if (h == 0) cb[w * (JT * 32) + tt * 32 + li] = tot;
       };
-#if LAKER_SYM_UNROLLTT
-      if (nt == V1_JT) {
-#pragma unroll
-        for (int tt = 0; tt < V1_JT; ++tt) tile(tt);
-      } else {
-        for (int tt = 0; tt < nt; ++tt) tile(tt);
-      }
-#else
       for (int tt = 0; tt < nt; ++tt) tile(tt);
-#endif
-#if LAKER_SYM_RACC
-#pragma unroll
-      for (int it = 0; it < ITW; ++it) acc64[it] += (double)(racc[it][0] + racc[it][1]);
-#endif
     }
     if (more) stash(odd ? buf0 : buf1);
     __syncthreads();
@@ -307,20 +235,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sym1_min_wa
       const int64_t t = tc + tt;
       const int64_t j = t * 32 + (threadIdx.x & 31);
       if (t >= tdiag && j < n) {
-        double v = 0.0;
-        for (int ww = 0; ww < nact; ++ww) v += (double)cb[ww * (V1_JT * 32) + threadIdx.x];
+        // only the last row block has idle waves, and it has no tile right of its diagonal block
+        float v = cb[threadIdx.x];
+#pragma unroll
+        for (int ww = 1; ww < NW; ++ww) v += cb[ww * (JT * 32) + threadIdx.x];
         part_c[sym_coff(b, n) + j - SYM_RB * (b + 1)] = v;
       }
     }
     odd = !odd;
   }
   if (!active) return;
+  // row partials: the 32 lanes of a half-wave hold the same 16 rows of a row tile
+  const int vq = ((lane >> 1) & 1) + 2 * ((lane >> 2) & 1) + 4 * ((lane >> 3) & 1) + 8 * ((lane >> 4) & 1);
+  const int jq = jj_of(vq, h);
 #pragma unroll
   for (int it = 0; it < ITW; ++it) {
-    double s = acc64[it];
-    s += __shfl_xor(s, 32, 64);
-    const int64_t i = I0 + it * 32 + li;
-    if (h == 0 && i < n) part_r[(int64_t)bs.y * n + i] = s;
+    const float s = transpose_reduce16(racc[it], lane);
+    const int64_t i = I0 + it * 32 + jq;
+    if ((lane & 1) == 0 && i < n) part_r[(int64_t)bs.y * n + i] = s;
   }
 }
 
@@ -329,7 +261,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sym1_min_wa
 // block's sum of x.y (q = 0), x.x (q = 1) and y.y (q = 2) over its rows, with
 // x = Xr and y the stored Y, in the order and with the block reduction of the
 // PCG's column-dot kernel for 256-row blocks (bit-identical partials).
-__global__ void mv_sym_reduce_kernel(const double* __restrict__ part_r, const double* __restrict__ part_c,
+__global__ void mv_sym_reduce_kernel(const SymPart* __restrict__ part_r, const SymPart* __restrict__ part_c,
                                      int64_t n, double lam, const float* __restrict__ Xr, int64_t ldxr,
                                      float* __restrict__ Y, int64_t ldy, int accumulate,
                                      const int* __restrict__ skip, double* __restrict__ dots) {
@@ -386,9 +318,10 @@ int launch_sym1(Operator& op, int mode, const void* colpack, const void* rowpack
                        int64_t ldxr, bool clamp, float clampv, int accumulate, const int* skip,
                        hipStream_t st, double* dots) {
   // the 16-bit modes (BF16 3-way split, FP16 2-way split) share one layout
-  const size_t chunk = mode == LAKER_PREC_FP32 ? V1Layout<LAKER_PREC_FP32>::chunk_bytes(ka)
-                                               : V1Layout<LAKER_PREC_MIXED>::chunk_bytes(ka);
-  const size_t lds = 2 * chunk + 2 * 4 * V1_JT * 32 * sizeof(float);
+  const size_t lbytes = mode == LAKER_PREC_FP32 ? V1Layout<LAKER_PREC_FP32>::lbytes(ka)
+                                                : V1Layout<LAKER_PREC_MIXED>::lbytes(ka);
+  const size_t chunk = (size_t)SYM_JT * 32 * (lbytes + 4);
+  const size_t lds = 2 * chunk + (2 * SYM_WAVES * SYM_JT * 32 + SYM_RB) * sizeof(float);
   if (lds > 65536 || op.sym_nwg <= 0) return 1;
   const int ns = mode == LAKER_PREC_FP32 ? ka / 2 : ka / 16;
   const bool fp32_ok = ns == 2 || ns == 3 || ns == 4 || ns == 5 || ns == 8 || ns == 16;
@@ -397,12 +330,12 @@ int launch_sym1(Operator& op, int mode, const void* colpack, const void* rowpack
   const int64_t ntiles = ceil_div(n, 32);
   const int64_t nsegmax = ceil_div(ntiles, SYM_SEG);
   const int64_t ncb = ceil_div(n, SYM_RB) - 1;  // row blocks with columns right of their diagonal block
-  LAKER_TRY(op.ensure_part((size_t)nsegmax * n * sizeof(double)));
-  LAKER_TRY(op.part_c.ensure((size_t)std::max<int64_t>(1, sym_coff(ncb, n)) * sizeof(double)));
-  double* part_r = op.part.as<double>();
-  double* part_c = op.part_c.as<double>();
+  LAKER_TRY(op.ensure_part((size_t)nsegmax * n * sizeof(SymPart)));
+  LAKER_TRY(op.part_c.ensure((size_t)std::max<int64_t>(1, sym_coff(ncb, n)) * sizeof(SymPart)));
+  SymPart* part_r = op.part.as<SymPart>();
+  SymPart* part_c = op.part_c.as<SymPart>();
   const int2* map = op.sym_map.as<int2>();
-  dim3 grid((unsigned)op.sym_nwg), block(256);
+  dim3 grid((unsigned)op.sym_nwg), block(64 * SYM_WAVES);
 #define LAKER_S1(MODE, NS)                                                                            \
   do {                                                                                                \
     if (clamp)                                                                                        \
@@ -441,7 +374,7 @@ int launch_sym1(Operator& op, int mode, const void* colpack, const void* rowpack
 #undef LAKER_S1
   LAKER_KCHECK("mv_sym1_kernel");
   hipLaunchKernelGGL(mv_sym_reduce_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st,
-                     (const double*)part_r, (const double*)part_c, n, lam, Xr, ldxr, Y, ldy, accumulate, skip,
+                     (const SymPart*)part_r, (const SymPart*)part_c, n, lam, Xr, ldxr, Y, ldy, accumulate, skip,
                      Xr ? dots : nullptr);
   LAKER_KCHECK("mv_sym_reduce_kernel");
   return LAKER_OK;

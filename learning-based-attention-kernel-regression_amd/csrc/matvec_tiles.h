@@ -155,14 +155,13 @@ struct V1Layout {
 
 // Layout constants of the symmetric k = 1 product (matvec_sym.hip); the
 // operator handle builds its workgroup map from them.
-#ifndef LAKER_SYM_ITW
-#define LAKER_SYM_ITW 4
-#endif
-constexpr int SYM_SEG = 32;                // column tiles per workgroup (1024 columns: 16..64 measured,
-                                           // 32 best for load balance against row-partial traffic)
-constexpr int SYM_ITW = LAKER_SYM_ITW;     // row tiles per wave
-constexpr int SYM_RB = 4 * 32 * SYM_ITW;   // rows per workgroup
-constexpr int SYM_DT = SYM_RB / 32;        // column tiles of the diagonal block
+constexpr int SYM_SEG = 32;     // column tiles per workgroup (1024 columns; 512 and 2048 measured
+                                // equal or slower: load balance against row-partial traffic)
+constexpr int SYM_WAVES = 8;    // waves per workgroup
+constexpr int SYM_ITW = 2;      // row tiles per wave
+constexpr int SYM_JT = 4;       // column tiles per LDS chunk
+constexpr int SYM_RB = SYM_WAVES * 32 * SYM_ITW;  // rows per workgroup (512)
+constexpr int SYM_DT = SYM_RB / 32;               // column tiles of the diagonal block
 
 __host__ __device__ inline int64_t sym_coff(int64_t b, int64_t n) { return b * n - (SYM_RB / 2) * b * (b + 1); }
 
