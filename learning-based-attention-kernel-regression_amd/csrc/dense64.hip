@@ -200,68 +200,74 @@ LAKER_INST_GEMM(double, float, float)
 // ---------------------------------------------------------------------------
 constexpr int PNB = 64;
 
-// Unblocked Cholesky of the nb x nb diagonal block at (k0,k0), in LDS, and
-// the inverse of the factor into dinv (64 x 64 row-major, identity-padded
-// beyond nb).  Sets *info = global index + 1 of the first non-positive pivot.
-__global__ __launch_bounds__(256) void potrf_diag_kernel(double* __restrict__ A, int64_t lda,
-                                                         int64_t k0, int nb, int* __restrict__ info,
-                                                         double* __restrict__ dinv) {
-  __shared__ double L[PNB][PNB + 1];
-  const int tid = threadIdx.x, c = tid & 63, rg = tid >> 6;
-  for (int i = rg; i < PNB; i += 4)
-    L[i][c] = (i < nb && c < nb) ? A[(k0 + i) * lda + k0 + c] : (i == c ? 1.0 : 0.0);
-  // pivot j: sqrt of the updated diagonal entry (a non-positive one is
-  // recorded in info and replaced, so the factorisation runs on without NaNs)
-  auto pivot = [&](int j) {
-    double v = L[j][j];
-    if (!(v > 0.0)) {
-      if (*info == 0) *info = (int)(k0 + j + 1);
-      v = 1e300;
-    }
-    L[j][j] = sqrt(v);
-  };
-  if (tid == 0 && nb > 0) pivot(0);
-  __syncthreads();
-  for (int j = 0; j < nb; ++j) {
-    const double rjj = 1.0 / L[j][j];
-    if (tid > j && tid < nb) L[tid][j] *= rjj;
-    __syncthreads();
-    // trailing update of the lower triangle within the block; thread
-    // (c = j + 1, rg = 0) applies the last update to entry (j + 1, j + 1) and
-    // takes that pivot at once (no other thread reads it in this phase)
-    if (c > j && c < nb) {
-      const double lcj = L[c][j];
-#pragma unroll 4
-      for (int i = j + 1 + rg; i < nb; i += 4)
-        if (c <= i) L[i][c] -= L[i][j] * lcj;
-    }
-    if (tid == j + 1 && j + 1 < nb) pivot(j + 1);
-    __syncthreads();
-  }
-  for (int i = rg; i < nb; i += 4)
-    if (c < nb) A[(k0 + i) * lda + k0 + c] = (c <= i) ? L[i][c] : 0.0;
-  // column t of L^{-1} by forward substitution (entries above t stay zero),
-  // column-oriented: once x_p is known, every later row i takes
-  // v_i -= L[i][p] x_p, so each v_i receives its terms in the order p = 0, 1, ...
-  // of the row-oriented dot product while the rows update in parallel.  Four
-  // threads (adjacent lanes of one wave) share column t = tid / 4; thread r
-  // keeps the rows i = 4m + r, and x_p comes from the lane that owns row p.
+// Unblocked Cholesky of the nb x nb diagonal block at (k0,k0) and the inverse
+// of its factor into dinv (64 x 64 row-major, identity-padded beyond nb), by
+// one wave: lane i holds row i in registers (the loops over columns are
+// unrolled, so every register index is fixed) and nothing branches on data.
+// Right-looking: at step k every lane scales its entry of column k by
+// 1 / sqrt(pivot), the column goes through LDS, and each lane applies
+// l_ik l_jk to its entries j > k, so every entry receives its updates in the
+// order k = 0, 1, ...  Sets *info = global index + 1 of the first
+// non-positive pivot (replaced by 1e300 so the factorisation runs on
+// without NaNs).
+__device__ __forceinline__ double lane_bcast(double v, int src) {
+  const int2 w = __builtin_bit_cast(int2, v);
+  return __builtin_bit_cast(double, int2{__builtin_amdgcn_readlane(w.x, src), __builtin_amdgcn_readlane(w.y, src)});
+}
+
+__global__ __launch_bounds__(64) void potrf_diag_kernel(double* __restrict__ A, int64_t lda, int64_t k0, int nb,
+                                                        int* __restrict__ info, double* __restrict__ dinv) {
+  __shared__ double col[2][PNB];             // column k of L, double-buffered across steps
+  __shared__ double Lc[PNB * (PNB + 1)];     // L column-major: Lc[c (PNB + 1) + r] = L[r][c]
+  const int i = threadIdx.x;
+  double a[PNB];
   {
-    const int t = tid >> 2, r = tid & 3, lane = tid & 63;
-    double v[PNB / 4];
+    const bool rin = i < nb;
+    const double* row = A + (k0 + (rin ? i : 0)) * lda + k0;
 #pragma unroll
-    for (int m = 0; m < PNB / 4; ++m) v[m] = (4 * m + r == t) ? 1.0 : 0.0;
-#pragma unroll
-    for (int p = 0; p < PNB; ++p) {
-      const int m0 = p >> 2;
-      double xo = v[m0] / L[p][p];
-      xo = __shfl(xo, (lane & ~3) | (p & 3), 64);
-      const double xp = (p < t) ? 0.0 : xo;
-      if (r == (p & 3)) dinv[p * PNB + t] = xp;
-#pragma unroll
-      for (int m = m0; m < PNB / 4; ++m)
-        if (4 * m + r > p) v[m] -= L[4 * m + r][p] * xp;
+    for (int c = 0; c < PNB; ++c) {
+      const double g = row[c < nb ? c : 0];
+      a[c] = (rin && c < nb) ? g : (i == c ? 1.0 : 0.0);
     }
+  }
+  int firstbad = -1;  // uniform: the pivot is read from lane k
+#pragma unroll
+  for (int k = 0; k < PNB; ++k) {
+    double piv = lane_bcast(a[k], k);
+    const bool bad = !(piv > 0.0);
+    if (bad && firstbad < 0 && k < nb) firstbad = k;
+    piv = bad ? 1e300 : piv;
+    const double d = sqrt(piv);
+    const double r = 1.0 / d;
+    a[k] = i == k ? d : (i > k ? a[k] * r : 0.0);
+    col[k & 1][i] = a[k];
+    Lc[k * (PNB + 1) + i] = a[k];
+    __syncthreads();
+#pragma unroll
+    for (int j = k + 1; j < PNB; ++j) a[j] -= a[k] * col[k & 1][j];
+  }
+  if (i == 0 && firstbad >= 0 && *info == 0) *info = (int)(k0 + firstbad + 1);
+  if (i < nb) {
+    double* row = A + (k0 + i) * lda + k0;
+#pragma unroll
+    for (int c = 0; c < PNB; ++c)
+      if (c < nb) row[c] = c <= i ? a[c] : 0.0;
+  }
+  // column t = i of L^{-1} by forward substitution, column-oriented: once x_p
+  // is known, every later row takes v_q -= L[q][p] x_p, so each v_q receives
+  // its terms in the order p = 0, 1, ... of the row-oriented dot product.
+  // v_p stays exactly zero for p < t (zero right-hand side, zero x so far),
+  // so x_p = v_p / L[p][p] is zero there without a test.
+  double v[PNB];
+#pragma unroll
+  for (int q = 0; q < PNB; ++q) v[q] = q == i ? 1.0 : 0.0;
+#pragma unroll
+  for (int p = 0; p < PNB; ++p) {
+    const double* lp = Lc + p * (PNB + 1);
+    const double xp = v[p] / lp[p];
+    dinv[p * PNB + i] = xp;
+#pragma unroll
+    for (int q = p + 1; q < PNB; ++q) v[q] -= lp[q] * xp;
   }
 }
 
@@ -301,7 +307,7 @@ int potrf_lower(double* A, int64_t lda, int64_t n, int* info_dev, DevBuf* ws, Tr
   for (int64_t k0 = 0; k0 < n; k0 += PNB) {
     const int nb = (int)std::min<int64_t>(PNB, n - k0);
     double* dk = dinv + (k0 / PNB) * PNB * PNB;
-    hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, st, A, lda, k0, nb, info_dev, dk);
+    hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(64), 0, st, A, lda, k0, nb, info_dev, dk);
     LAKER_KCHECK("potrf_diag_kernel");
     const int64_t rem = n - k0 - nb;
     if (rem > 0) {

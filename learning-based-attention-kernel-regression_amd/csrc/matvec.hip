@@ -770,7 +770,8 @@ int laker_operator_create(const void* E, int64_t n, int32_t d, int32_t dtype, do
   std::vector<int2> smap;
   {
     // segment-major order: consecutive workgroups (spread over the XCDs)
-    // work on the same column segment
+    // work on the same column segment (each XCD taking a contiguous run of
+    // the list measured 1 % slower)
     const int64_t ntiles = ceil_div(n, 32), nb = ceil_div(n, SYM_RB), nseg = ceil_div(ntiles, SYM_SEG);
     for (int64_t sg = 0; sg < nseg; ++sg)
       for (int64_t b = 0; b < nb && SYM_DT * b < SYM_SEG * (sg + 1); ++b) smap.push_back(int2{(int)b, (int)sg});
