@@ -243,6 +243,65 @@ __global__ void sum_parts_kernel(const double* __restrict__ part, int nb, double
   }
 }
 
+// Device state of the Newton-Schulz loop: status 0 running, 1 converged,
+// 2 failed (NaN).
+struct NsState {
+  double prev_err;
+  int status;
+  int iterations;
+};
+
+__global__ void ns_init_kernel(NsState* s) {
+  s->prev_err = 1e300;
+  s->status = 0;
+  s->iterations = 0;
+}
+
+// The stopping rule of one Newton-Schulz step from the partials of
+// err^2 = ||Z Y - I||_F^2 (summed in block order): stop once err < tol_abs,
+// or once the error stalls below 1e-10 (err > prev_err / 2).
+__global__ void ns_check_kernel(const double* __restrict__ part, int nb, double tol_abs, NsState* s) {
+  if (threadIdx.x != 0 || s->status != 0) return;
+  double e2 = 0.0;
+  for (int b = 0; b < nb; ++b) e2 += part[b];
+  const double err = sqrt(e2);
+  if (!(err == err)) {
+    s->status = 2;
+  } else if (err < tol_abs || (err > 0.5 * s->prev_err && err < 1e-10)) {
+    s->status = 1;
+  } else {
+    s->prev_err = err;
+    s->iterations += 1;
+  }
+}
+
+// T = I once the loop has stopped: the steps launched after the stopping
+// point then leave Y and Z exactly as they were.
+__global__ void ns_hold_kernel(double* __restrict__ T, int64_t n, const NsState* __restrict__ s) {
+  if (s->status == 0) return;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  T[idx] = (idx / n == idx % n) ? 1.0 : 0.0;
+}
+
+// Y = M / sqrt(fro2)
+__global__ void scale_by_frob_kernel(double* __restrict__ M, int64_t n, const double* __restrict__ fro2) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  const double f = 1.0 / sqrt(*fro2);
+  M[idx] = M[idx] * f;
+}
+
+// S[i][j] = (Z[i][j] + Z[j][i]) / (2 sqrt(sqrt(fro2))) - g delta_ij
+__global__ void sym_scale_frob_kernel(const double* __restrict__ Zm, double* __restrict__ S, int64_t n,
+                                      const double* __restrict__ fro2, double g) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  const int64_t i = idx / n, j = idx % n;
+  const double f = 1.0 / sqrt(sqrt(*fro2));
+  S[idx] = 0.5 * f * (Zm[idx] + Zm[j * n + i]) - (i == j ? g : 0.0);
+}
+
 // S = Z * f - g I
 __global__ void scale_sub_diag_kernel(const double* __restrict__ Zm, double* __restrict__ S, int64_t n,
                                       double f, double g) {
@@ -738,13 +797,18 @@ __global__ __launch_bounds__(256) void qw8h_kernel(const _Float16* __restrict__ 
 // holds the split's sums; gram_reduce_kernel adds the splits in order.
 constexpr int GH_P = 40;  // LDS pitch (halves) of one transposed column: 32 rows + 8
 __global__ __launch_bounds__(256) void gram_f16_kernel(const _Float16* __restrict__ Q, int64_t n, int nr,
-                                                       int64_t rows_per_split, const int2* __restrict__ pairs,
-                                                       double* __restrict__ part) {
+                                                       int64_t rows_per_split, double* __restrict__ part) {
   __shared__ __align__(16) _Float16 ta[64 * GH_P];
   __shared__ __align__(16) _Float16 tb[64 * GH_P];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, wi = w >> 1, wj = w & 1, h = lane >> 5;
-  const int2 tp = pairs[blockIdx.x];
-  const int c0 = 64 * tp.x, c1 = 64 * tp.y;
+  // blockIdx.x enumerates the tile pairs (ta, tb), ta <= tb, row by row
+  const int nt = (nr + 63) / 64;
+  int ta_i = 0, rem = (int)blockIdx.x;
+  while (rem >= nt - ta_i) {
+    rem -= nt - ta_i;
+    ++ta_i;
+  }
+  const int c0 = 64 * ta_i, c1 = 64 * (ta_i + rem);
   const int64_t s = blockIdx.y;
   const int64_t r0 = s * rows_per_split;
   const int64_t r1 = (r0 + rows_per_split < n) ? r0 + rows_per_split : n;
@@ -1017,31 +1081,27 @@ int precond_prepare_half(Precond& pc, hipStream_t st) {
   LAKER_KCHECK("to_half_kernel");
   DevBuf G, Li, T, info;
   TriWork tw;
-  LAKER_TRY(G.ensure(nn * 8));
-  LAKER_TRY(Li.ensure(nn * 8));
-  LAKER_TRY(T.ensure(nn * 8));
-  LAKER_TRY(info.ensure(sizeof(int)));
+  tw.bind(st);
+  LAKER_TRY(G.ensure(nn * 8, st));
+  LAKER_TRY(Li.ensure(nn * 8, st));
+  LAKER_TRY(T.ensure(nn * 8, st));
+  LAKER_TRY(info.ensure(sizeof(int), st));
   LAKER_TRY(pc.Sh.ensure(nn * 8));
   // Qh^T Qh on FP16 MFMA (exact products, FP32 over 256 rows, FP64 beyond)
   {
     const int nt = (int)ceil_div(nr, 64);
-    std::vector<int2> pairs;
-    for (int a = 0; a < nt; ++a)
-      for (int b = a; b < nt; ++b) pairs.push_back(int2{a, b});
+    const int npairs = nt * (nt + 1) / 2;
     const int64_t nsplit = std::max<int64_t>(1, std::min<int64_t>(64, ceil_div(n, 2048)));
     const int64_t rps = round_up(ceil_div(n, nsplit), 32);
     const int64_t ns_eff = ceil_div(n, rps);
-    DevBuf pairs_d, gpart;
-    LAKER_TRY(pairs_d.ensure(pairs.size() * sizeof(int2)));
-    LAKER_TRY(gpart.ensure((size_t)ns_eff * nn * 8));
-    LAKER_HIP_CHECK(hipMemcpyAsync(pairs_d.ptr, pairs.data(), pairs.size() * sizeof(int2), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(gram_f16_kernel, dim3((unsigned)pairs.size(), (unsigned)ns_eff), dim3(256), 0, st,
-                       pc.Qh.as<_Float16>(), n, nr, rps, pairs_d.as<int2>(), gpart.as<double>());
+    DevBuf gpart;
+    LAKER_TRY(gpart.ensure((size_t)ns_eff * nn * 8, st));
+    hipLaunchKernelGGL(gram_f16_kernel, dim3((unsigned)npairs, (unsigned)ns_eff), dim3(256), 0, st,
+                       pc.Qh.as<_Float16>(), n, nr, rps, gpart.as<double>());
     LAKER_KCHECK("gram_f16_kernel");
     hipLaunchKernelGGL(gram_reduce_kernel, dim3((unsigned)ceil_div((int64_t)nn, 256)), dim3(256), 0, st,
                        (const double*)gpart.as<double>(), (int)ns_eff, nr, G.as<double>());
     LAKER_KCHECK("gram_reduce_kernel");
-    LAKER_HIP_CHECK(hipStreamSynchronize(st));  // pairs (host vector) and gpart are freed on scope exit
   }
   LAKER_HIP_CHECK(hipMemsetAsync(info.ptr, 0, sizeof(int), st));
   LAKER_TRY(potrf_lower(G.as<double>(), nr, nr, info.as<int>(), &pc.ws, tw, st));
@@ -1093,12 +1153,12 @@ int precond_apply_half(Precond& pc, const float* X, float* Z, hipStream_t st) {
   return LAKER_OK;
 }
 
-// One Cholesky-QR pass in FP64: G = V^T V (+ shift I), L = chol(G), V <- V L^{-T}, Racc <- L^T Racc.
-// Returns LAKER_ENUMERIC if the factorisation failed.
-static int cholqr_pass(double* V, int64_t n, int nr, double shift, double* G, double* Linv,
-                       double* Racc, double* tmp, double* Vtmp, int* info, DevBuf* ws, TriWork& tw,
-                       hipStream_t st) {
-  LAKER_TRY((gemm<double, double, double>(true, false, nr, nr, n, 1.0, V, nr, V, nr, 0.0, G, nr, ws, st)));
+// One Cholesky-QR pass in FP64: G = Vin^T Vin (+ shift I), L = chol(G),
+// Vout = Vin L^{-T}, Racc <- L^T Racc.  A failed factorisation leaves a
+// non-zero *info (device); the caller reads it.
+static int cholqr_pass(const double* Vin, double* Vout, int64_t n, int nr, double shift, double* G, double* Linv,
+                       double* Racc, double* tmp, int* info, DevBuf* ws, TriWork& tw, hipStream_t st) {
+  LAKER_TRY((gemm<double, double, double>(true, false, nr, nr, n, 1.0, Vin, nr, Vin, nr, 0.0, G, nr, ws, st)));
   if (shift > 0.0) {
     hipLaunchKernelGGL(add_diag_kernel, dim3((unsigned)ceil_div(nr, 256)), dim3(256), 0, st, G, (int64_t)nr,
                        shift);
@@ -1106,16 +1166,12 @@ static int cholqr_pass(double* V, int64_t n, int nr, double shift, double* G, do
   }
   LAKER_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int), st));
   LAKER_TRY(potrf_lower(G, nr, nr, info, ws, tw, st));
-  int hinfo = 0;
-  LAKER_TRY(read_scalar(info, &hinfo, sizeof(int), st));
-  if (hinfo != 0) return LAKER_ENUMERIC;
   // Linv = L^{-1}
   LAKER_TRY(set_identity(Linv, nr, 1.0, st));
   LAKER_TRY(trsm_lower_left(G, nr, nr, Linv, nr, nr, ws, tw, st));
-  // Vtmp = V Linv^T ; copy back
-  LAKER_TRY((gemm<double, double, double>(false, true, n, nr, nr, 1.0, V, nr, Linv, nr, 0.0, Vtmp, nr,
+  // Vout = Vin Linv^T
+  LAKER_TRY((gemm<double, double, double>(false, true, n, nr, nr, 1.0, Vin, nr, Linv, nr, 0.0, Vout, nr,
                                            nullptr, st)));
-  LAKER_HIP_CHECK(hipMemcpyAsync(V, Vtmp, (size_t)n * nr * 8, hipMemcpyDeviceToDevice, st));
   // Racc <- L^T Racc   (L^T upper = R factor of this pass)
   LAKER_TRY((gemm<double, double, double>(true, false, nr, nr, nr, 1.0, G, nr, Racc, nr, 0.0, tmp, nr,
                                            nullptr, st)));
@@ -1127,17 +1183,23 @@ static int cholqr_pass(double* V, int64_t n, int nr, double shift, double* G, do
 // Cholesky-QR2 (two passes), or a shifted Cholesky-QR3 when the first Gram
 // matrix is not numerically positive definite.  V receives the basis, Racc the
 // accumulated upper-triangular factor (src = V Racc).  LAKER_ENUMERIC if every
-// route fails (rank-deficient block).
+// route fails (rank-deficient block).  The two passes of Cholesky-QR2 run
+// without a host round trip; their factorisation flags are read once at the
+// end (a failed first pass makes the second one's output meaningless, and the
+// shifted route then starts over from src).
 static int cholqr_block(const void* src, int dtype, int64_t n, int nr, double* V, double* Racc, DevBuf* ws,
                         hipStream_t st) {
   const int64_t nn = (int64_t)nr * nr;
   DevBuf Vtmp, G, Linv, tmp, infob;
   TriWork tri;
-  LAKER_TRY(Vtmp.ensure((size_t)n * nr * 8));
-  LAKER_TRY(G.ensure(nn * 8));
-  LAKER_TRY(Linv.ensure(nn * 8));
-  LAKER_TRY(tmp.ensure(nn * 8));
-  LAKER_TRY(infob.ensure(sizeof(int)));
+  tri.bind(st);
+  LAKER_TRY(Vtmp.ensure((size_t)n * nr * 8, st));
+  LAKER_TRY(G.ensure(nn * 8, st));
+  LAKER_TRY(Linv.ensure(nn * 8, st));
+  LAKER_TRY(tmp.ensure(nn * 8, st));
+  LAKER_TRY(infob.ensure(3 * sizeof(int), st));
+  int* info = infob.as<int>();
+  double* Vt = Vtmp.as<double>();
   auto load_v = [&]() -> int {
     if (dtype == LAKER_F32)
       hipLaunchKernelGGL((convert_kernel<float, double>), dim3((unsigned)ceil_div(n * nr, 256)), dim3(256), 0,
@@ -1147,23 +1209,32 @@ static int cholqr_block(const void* src, int dtype, int64_t n, int nr, double* V
     LAKER_HIP_CHECK(hipGetLastError());
     return set_identity(Racc, nr, 1.0, st);
   };
-  auto pass = [&](double shift) {
-    return cholqr_pass(V, n, nr, shift, G.as<double>(), Linv.as<double>(), Racc, tmp.as<double>(),
-                       Vtmp.as<double>(), infob.as<int>(), ws, tri, st);
+  auto pass = [&](const double* vin, double* vout, double shift, int q) {
+    return cholqr_pass(vin, vout, n, nr, shift, G.as<double>(), Linv.as<double>(), Racc, tmp.as<double>(),
+                       info + q, ws, tri, st);
+  };
+  auto read_info = [&](int* h, int count) -> int {
+    LAKER_HIP_CHECK(hipMemcpyAsync(h, info, count * sizeof(int), hipMemcpyDeviceToHost, st));
+    LAKER_HIP_CHECK(hipStreamSynchronize(st));
+    return LAKER_OK;
   };
   LAKER_TRY(load_v());
-  int rc = pass(0.0);
-  if (rc == LAKER_ENUMERIC) {
-    // shifted Cholesky-QR: shift = 11 (n nr + nr (nr + 1)) u ||U||_F^2, ||U||_F^2 <= nr
-    // (columns of unit norm; callers normalise first)
-    LAKER_TRY(load_v());
-    const double u = 1.1102230246251565e-16;
-    const double shift = 11.0 * ((double)n * nr + (double)nr * (nr + 1)) * u * (double)nr;
-    rc = pass(shift);
-    if (rc == LAKER_OK) rc = pass(0.0);
-  }
-  if (rc == LAKER_OK) rc = pass(0.0);
-  return rc;
+  LAKER_TRY(pass(V, Vt, 0.0, 0));
+  LAKER_TRY(pass(Vt, V, 0.0, 1));
+  int hinfo[3] = {0, 0, 0};
+  LAKER_TRY(read_info(hinfo, 2));
+  if (hinfo[0] == 0) return hinfo[1] == 0 ? LAKER_OK : LAKER_ENUMERIC;
+  // shifted Cholesky-QR: shift = 11 (n nr + nr (nr + 1)) u ||U||_F^2, ||U||_F^2 <= nr
+  // (columns of unit norm; callers normalise first), then two plain passes
+  LAKER_TRY(load_v());
+  const double u = 1.1102230246251565e-16;
+  const double shift = 11.0 * ((double)n * nr + (double)nr * (nr + 1)) * u * (double)nr;
+  LAKER_TRY(pass(V, Vt, shift, 0));
+  LAKER_TRY(pass(Vt, V, 0.0, 1));
+  LAKER_TRY(pass(V, Vt, 0.0, 2));
+  LAKER_HIP_CHECK(hipMemcpyAsync(V, Vt, (size_t)n * nr * 8, hipMemcpyDeviceToDevice, st));
+  LAKER_TRY(read_info(hinfo, 3));
+  return (hinfo[0] == 0 && hinfo[1] == 0 && hinfo[2] == 0) ? LAKER_OK : LAKER_ENUMERIC;
 }
 
 // Steps 2-5 of the build on probe responses U (n x nr, op dtype; consumed).
@@ -1176,7 +1247,7 @@ static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, i
   (void)op;
   // 2. column normalisation in the solver dtype
   DevBuf ss;
-  LAKER_TRY(ss.ensure((size_t)nr * 8));
+  LAKER_TRY(ss.ensure((size_t)nr * 8, st));
   if (pc->dtype == LAKER_F32) {
     hipLaunchKernelGGL((colsumsq_kernel<float>), dim3(nr), dim3(256), 0, st, U.as<float>(), n, nr, ss.as<double>());
     LAKER_KCHECK("colsumsq_kernel");
@@ -1194,8 +1265,8 @@ static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, i
 
   // 3. Cholesky-QR2 in FP64 (shifted Cholesky-QR3 on failure)
   DevBuf V, Racc;
-  LAKER_TRY(V.ensure((size_t)n * nr * 8));
-  LAKER_TRY(Racc.ensure(nn * 8));
+  LAKER_TRY(V.ensure((size_t)n * nr * 8, st));
+  LAKER_TRY(Racc.ensure(nn * 8, st));
   {
     const int rc = cholqr_block(U.ptr, pc->dtype, n, nr, V.as<double>(), Racc.as<double>(), &pc->ws, st);
     if (rc == LAKER_ENUMERIC) return fail(LAKER_ENUMERIC, "CCCP build: Cholesky-QR of the probe responses failed");
@@ -1228,17 +1299,18 @@ static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, i
   const double a_rho = (1.0 - rho) * gamma / s + rho;
   DevBuf Cb, Cn, Mb, Zb, Rw, Hb, scb, partb, infob, Linv;
   TriWork tri;
-  LAKER_TRY(infob.ensure(sizeof(int)));
-  LAKER_TRY(Linv.ensure(nn * 8));
-  LAKER_TRY(Cb.ensure(nn * 8));
-  LAKER_TRY(Cn.ensure(nn * 8));
-  LAKER_TRY(Mb.ensure(nn * 8));
-  LAKER_TRY(Zb.ensure(nn * 8));
-  LAKER_TRY(Rw.ensure(nn * 8));
-  LAKER_TRY(Hb.ensure(nn * 8));
-  LAKER_TRY(scb.ensure(sizeof(CccpScalars)));
+  tri.bind(st);
+  LAKER_TRY(infob.ensure(sizeof(int), st));
+  LAKER_TRY(Linv.ensure(nn * 8, st));
+  LAKER_TRY(Cb.ensure(nn * 8, st));
+  LAKER_TRY(Cn.ensure(nn * 8, st));
+  LAKER_TRY(Mb.ensure(nn * 8, st));
+  LAKER_TRY(Zb.ensure(nn * 8, st));
+  LAKER_TRY(Rw.ensure(nn * 8, st));
+  LAKER_TRY(Hb.ensure(nn * 8, st));
+  LAKER_TRY(scb.ensure(sizeof(CccpScalars), st));
   const int upd_blocks = (int)std::min<int64_t>(256, ceil_div(nn, 256));
-  LAKER_TRY(partb.ensure((size_t)upd_blocks * 2 * 8));
+  LAKER_TRY(partb.ensure((size_t)upd_blocks * 2 * 8, st));
   LAKER_HIP_CHECK(hipMemsetAsync(Cb.ptr, 0, nn * 8, st));
   double a = 1.0;
   int it_done = 0;
@@ -1285,25 +1357,30 @@ static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, i
   LAKER_TRY(pc->C.ensure(nn * 8));
   LAKER_HIP_CHECK(hipMemcpyAsync(pc->C.ptr, Cb.ptr, nn * 8, hipMemcpyDeviceToDevice, st));
 
-  // 5. S = M^{-1/2} - a^{-1/2} I by coupled Newton-Schulz on M / c
+  // 5. S = M^{-1/2} - a^{-1/2} I by coupled Newton-Schulz on M / c, c = ||M||_F.
+  // The stopping test runs on the device (ns_check_kernel) and the host reads
+  // its verdict every NS_POLL steps; steps after the stop multiply by I.
   hipLaunchKernelGGL(form_m_kernel, dim3((unsigned)ceil_div(nn, 256)), dim3(256), 0, st, Cb.as<double>(),
                      Mb.as<double>(), (int64_t)nr, a);
   LAKER_KCHECK("form_m_kernel");
-  hipLaunchKernelGGL(frob2_kernel, dim3(1), dim3(1024), 0, st, Mb.as<double>(), nn, scb.as<double>());
+  double* fro2 = scb.as<double>();
+  hipLaunchKernelGGL(frob2_kernel, dim3(1), dim3(1024), 0, st, Mb.as<double>(), nn, fro2);
   LAKER_KCHECK("frob2_kernel");
-  LAKER_HIP_CHECK(hipGetLastError());
-  double fro2 = 0.0;
-  LAKER_TRY(read_scalar(scb.ptr, &fro2, 8, st));
-  const double cn = std::sqrt(fro2);
   // Y = M / c  (in Mb), Z = I (in Zb)
-  hipLaunchKernelGGL(scale_sub_diag_kernel, dim3((unsigned)ceil_div(nn, 256)), dim3(256), 0, st,
-                     Mb.as<double>(), Mb.as<double>(), (int64_t)nr, 1.0 / cn, 0.0);
-  LAKER_KCHECK("scale_sub_diag_kernel");
+  hipLaunchKernelGGL(scale_by_frob_kernel, dim3((unsigned)ceil_div(nn, 256)), dim3(256), 0, st, Mb.as<double>(),
+                     (int64_t)nr, (const double*)fro2);
+  LAKER_KCHECK("scale_by_frob_kernel");
   LAKER_TRY(set_identity(Zb.as<double>(), nr, 1.0, st));
   const int ns_blocks = (int)std::min<int64_t>(256, ceil_div(nn, 256));
-  LAKER_TRY(partb.ensure((size_t)std::max(ns_blocks, upd_blocks) * 2 * 8));
-  double prev_err = 1e300;
-  bool ns_ok = false;
+  DevBuf nsb;
+  LAKER_TRY(nsb.ensure(sizeof(NsState), st));
+  NsState* nss = nsb.as<NsState>();
+  LAKER_TRY(partb.ensure((size_t)std::max(ns_blocks, upd_blocks) * 2 * 8, st));
+  hipLaunchKernelGGL(ns_init_kernel, dim3(1), dim3(1), 0, st, nss);
+  LAKER_KCHECK("ns_init_kernel");
+  constexpr int NS_POLL = 4;
+  const double ns_tol = 1e-13 * std::sqrt((double)nr);
+  NsState hns{};
   for (int it = 0; it < 200; ++it) {
     // P = Z Y -> Hb ; T -> Rw ; Y <- Y T (Cn) ; Z <- T Z (Linv)
     LAKER_TRY((gemm<double, double, double>(false, false, nr, nr, nr, 1.0, Zb.as<double>(), nr,
@@ -1311,34 +1388,29 @@ static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, i
     hipLaunchKernelGGL(ns_t_kernel, dim3(ns_blocks), dim3(256), 0, st, Hb.as<double>(), Rw.as<double>(),
                        (int64_t)nr, partb.as<double>());
     LAKER_KCHECK("ns_t_kernel");
-    hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(64), 0, st, partb.as<double>(), ns_blocks,
-                       scb.as<double>());
-    LAKER_KCHECK("sum_parts_kernel");
-    LAKER_HIP_CHECK(hipGetLastError());
-    double err2 = 0.0;
-    LAKER_TRY(read_scalar(scb.ptr, &err2, 8, st));
-    const double err = std::sqrt(err2);
-    if (!(err == err)) break;  // NaN
-    if (err < 1e-13 * std::sqrt((double)nr) || (err > 0.5 * prev_err && err < 1e-10)) {
-      ns_ok = true;
-      break;
-    }
-    prev_err = err;
+    hipLaunchKernelGGL(ns_check_kernel, dim3(1), dim3(64), 0, st, (const double*)partb.as<double>(), ns_blocks,
+                       ns_tol, nss);
+    LAKER_KCHECK("ns_check_kernel");
+    hipLaunchKernelGGL(ns_hold_kernel, dim3((unsigned)ceil_div(nn, 256)), dim3(256), 0, st, Rw.as<double>(),
+                       (int64_t)nr, (const NsState*)nss);
+    LAKER_KCHECK("ns_hold_kernel");
     LAKER_TRY((gemm<double, double, double>(false, false, nr, nr, nr, 1.0, Mb.as<double>(), nr,
                                              Rw.as<double>(), nr, 0.0, Cn.as<double>(), nr, &pc->ws, st)));
     LAKER_TRY((gemm<double, double, double>(false, false, nr, nr, nr, 1.0, Rw.as<double>(), nr,
                                              Zb.as<double>(), nr, 0.0, Linv.as<double>(), nr, &pc->ws, st)));
-    LAKER_HIP_CHECK(hipMemcpyAsync(Mb.ptr, Cn.ptr, nn * 8, hipMemcpyDeviceToDevice, st));
-    LAKER_HIP_CHECK(hipMemcpyAsync(Zb.ptr, Linv.ptr, nn * 8, hipMemcpyDeviceToDevice, st));
+    Mb.swap(Cn);
+    Zb.swap(Linv);
+    if ((it + 1) % NS_POLL == 0 || it == 199) {
+      LAKER_TRY(read_scalar(nss, &hns, sizeof(hns), st));
+      if (hns.status != 0) break;
+    }
   }
-  if (!ns_ok) return fail(LAKER_ENUMERIC, "CCCP build: Newton-Schulz inverse square root did not converge");
+  if (hns.status != 1) return fail(LAKER_ENUMERIC, "CCCP build: Newton-Schulz inverse square root did not converge");
   LAKER_TRY(pc->S.ensure(nn * 8));
   // S = (Z + Z^T)/(2 sqrt(c)) - a^{-1/2} I
-  hipLaunchKernelGGL(sym_scale_kernel, dim3((unsigned)ceil_div(nn, 256)), dim3(256), 0, st, Zb.as<double>(),
-                     pc->S.as<double>(), (int64_t)nr, 1.0 / std::sqrt(cn), pc->inv_sqrt_a);
-  LAKER_KCHECK("sym_scale_kernel");
-  LAKER_HIP_CHECK(hipGetLastError());
-  LAKER_HIP_CHECK(hipStreamSynchronize(st));
+  hipLaunchKernelGGL(sym_scale_frob_kernel, dim3((unsigned)ceil_div(nn, 256)), dim3(256), 0, st, Zb.as<double>(),
+                     pc->S.as<double>(), (int64_t)nr, (const double*)fro2, pc->inv_sqrt_a);
+  LAKER_KCHECK("sym_scale_frob_kernel");
   if (iterations) *iterations = it_done;
   return LAKER_OK;
 }
@@ -1467,7 +1539,7 @@ int laker_precond_build(laker_operator_t oh, int32_t nr, double gamma, double ep
   // so that product needs two BF16 products instead of three
   const bool xbf16 = !Rp && op.dtype == LAKER_F32 && probe_precision == LAKER_PREC_HILO;
   if (!Rp) {
-    LAKER_TRY(probe_buf.ensure((size_t)n * nr * esz));
+    LAKER_TRY(probe_buf.ensure((size_t)n * nr * esz, st));
     if (op.dtype == LAKER_F32)
       hipLaunchKernelGGL((randn_kernel<float>), dim3((unsigned)ceil_div(n * nr, 256)), dim3(256), 0, st,
                          probe_buf.as<float>(), n * nr, seed, xbf16 ? 1 : 0);
@@ -1477,7 +1549,7 @@ int laker_precond_build(laker_operator_t oh, int32_t nr, double gamma, double ep
     LAKER_KCHECK("randn_kernel");
     Rp = probe_buf.ptr;
   }
-  LAKER_TRY(U.ensure((size_t)n * nr * esz));
+  LAKER_TRY(U.ensure((size_t)n * nr * esz, st));
   const bool clamp = op.clamp_for(LAKER_CLAMP_AUTO);
   if (op.dtype == LAKER_F32) {
     int prec = probe_precision;
@@ -1512,7 +1584,7 @@ int laker_precond_build_from_responses(const void* U_in, int64_t n, int32_t nr, 
   pc->dtype = dtype;
   const size_t esz = dtype == LAKER_F32 ? 4 : 8;
   DevBuf U;
-  LAKER_TRY(U.ensure((size_t)n * nr * esz));
+  LAKER_TRY(U.ensure((size_t)n * nr * esz, st));
   LAKER_HIP_CHECK(hipMemcpyAsync(U.ptr, U_in, (size_t)n * nr * esz, hipMemcpyDeviceToDevice, st));
   Operator dummy;
   int its = 0;
@@ -1580,10 +1652,11 @@ int laker_orthonormalize(void* X, int64_t n, int32_t k, int32_t dtype, hipStream
   const int64_t nk = n * k;
   const dim3 eg((unsigned)ceil_div(nk, 256)), eb(256);
   DevBuf W, ss, V, Racc, ws;
-  LAKER_TRY(W.ensure((size_t)nk * 8));
-  LAKER_TRY(ss.ensure((size_t)k * 8));
-  LAKER_TRY(V.ensure((size_t)nk * 8));
-  LAKER_TRY(Racc.ensure((size_t)k * k * 8));
+  LAKER_TRY(W.ensure((size_t)nk * 8, st));
+  LAKER_TRY(ss.ensure((size_t)k * 8, st));
+  LAKER_TRY(V.ensure((size_t)nk * 8, st));
+  LAKER_TRY(Racc.ensure((size_t)k * k * 8, st));
+  ws.bind(st);
   // FP64 copy with unit columns (Q is unchanged by a positive column scaling)
   if (dtype == LAKER_F32)
     hipLaunchKernelGGL((convert_kernel<float, double>), eg, eb, 0, st, (const float*)X, W.as<double>(), nk);

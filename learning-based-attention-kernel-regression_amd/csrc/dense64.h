@@ -16,6 +16,10 @@ int gemm(bool tra, bool trb, int64_t M, int64_t N, int64_t K, double alpha, cons
 struct TriWork {
   DevBuf dinv;
   DevBuf panel;
+  void bind(hipStream_t st) {
+    dinv.bind(st);
+    panel.bind(st);
+  }
 };
 
 // In-place lower Cholesky of the n x n matrix A (upper triangle zeroed).

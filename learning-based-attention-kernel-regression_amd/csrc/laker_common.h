@@ -64,32 +64,54 @@ __device__ __host__ inline float bf16_bits_to_f32(uint16_t b) {
   return __builtin_bit_cast(float, (uint32_t)b << 16);
 }
 
-// Device memory owned by library handles.
+// Device memory of the library.  An unbound buffer (ensure(bytes) on an
+// empty buffer) is plain hipMalloc / hipFree memory.  ensure(bytes, stream)
+// on an empty buffer binds it to that stream: its block comes from, and on
+// release goes back to, a cache of blocks that only the same stream reuses
+// (stream order makes that safe without any host wait).  A bound buffer must
+// be used on its stream only -- temporaries inside one library call; a buffer
+// used on a second stream becomes unbound (plain hipFree on release).
+// Handle-owned buffers stay unbound.
 struct DevBuf {
   void* ptr = nullptr;
   size_t bytes = 0;
+  hipStream_t tag = nullptr;
+  bool tagged = false;
+  int dev = 0;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), bytes(o.bytes) {
+  DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), bytes(o.bytes), tag(o.tag), tagged(o.tagged), dev(o.dev) {
     o.ptr = nullptr;
     o.bytes = 0;
+    o.tagged = false;
   }
   DevBuf& operator=(DevBuf&& o) noexcept {
     if (this != &o) {
       release();
       ptr = o.ptr;
       bytes = o.bytes;
+      tag = o.tag;
+      tagged = o.tagged;
+      dev = o.dev;
       o.ptr = nullptr;
       o.bytes = 0;
+      o.tagged = false;
     }
     return *this;
   }
   void swap(DevBuf& o) noexcept {
     std::swap(ptr, o.ptr);
     std::swap(bytes, o.bytes);
+    std::swap(tag, o.tag);
+    std::swap(tagged, o.tagged);
+    std::swap(dev, o.dev);
   }
   int ensure(size_t want);
+  int ensure(size_t want, hipStream_t st);
+  // Marks an empty buffer as a temporary of stream st: later ensure() calls
+  // without a stream (helpers handed the buffer) allocate it bound to st.
+  void bind(hipStream_t st);
   void release();
   ~DevBuf() { release(); }
   template <class T> T* as() const { return reinterpret_cast<T*>(ptr); }

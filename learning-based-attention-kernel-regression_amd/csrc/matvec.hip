@@ -752,7 +752,7 @@ int laker_operator_create(const void* E, int64_t n, int32_t d, int32_t dtype, do
   }
   // skip-clamp predicate: max ||e||^2 (times scale) < limit
   DevBuf mx;
-  if ((rc = mx.ensure(8)) != LAKER_OK) {
+  if ((rc = mx.ensure(8, stream)) != LAKER_OK) {
     delete op;
     return rc;
   }

@@ -712,7 +712,7 @@ __global__ void frob2_part_kernel(const double* __restrict__ a, int64_t count, d
 
 static int frob_norm(const double* a, int64_t count, DevBuf& tmp, double* out, hipStream_t st) {
   const int nb = (int)std::min<int64_t>(256, ceil_div(count, 256));
-  LAKER_TRY(tmp.ensure((size_t)nb * 8));
+  LAKER_TRY(tmp.ensure((size_t)nb * 8, st));
   hipLaunchKernelGGL(frob2_part_kernel, dim3(nb), dim3(256), 0, st, a, count, tmp.as<double>());
   LAKER_KCHECK("frob2_part_kernel");
   LAKER_HIP_CHECK(hipGetLastError());
