@@ -179,10 +179,10 @@ __host__ __device__ constexpr int m2_chunk_bytes(int ns, int nparts, int ncc, in
 
 // Column operand rows and packed X operands of JT column tiles are staged
 // global -> registers -> LDS one chunk ahead (double buffer, one barrier per
-// chunk) and shared by the block's 4 waves.  GM: Gram operands, the BF16
+// chunk) and shared by the block's NW waves.  GM: Gram operands, the BF16
 // 3-way split (LAKER_PREC_MIXED) or the FP16 2-way split (GRAM_F16S, square
 // operator only).
-template <int GM, int NS, int GX, int NCC, int JT, int ITW, bool CLAMP>
+template <int GM, int NS, int GX, int NCC, int JT, int ITW, bool CLAMP, int NW = 4>
 __device__ __forceinline__ void mv_multi2_kernel_body(
     const void* __restrict__ colpack, const void* __restrict__ rowpack, int ka, int64_t n_rows,
     int64_t n_cols, const bf16x8* __restrict__ xh, const bf16x8* __restrict__ xl,
@@ -194,13 +194,14 @@ __device__ __forceinline__ void mv_multi2_kernel_body(
   constexpr int VA = JT * 32 * vpr;           // operand-row vectors per chunk
   constexpr int XT = NP * NCC * 2 * 64;       // X vectors per tile
   constexpr int VX = JT * XT;
-  constexpr int VPT = (VA + VX + 255) / 256;  // vectors per thread per chunk
+  constexpr int NT = 64 * NW;                 // threads of the block
+  constexpr int VPT = (VA + VX + NT - 1) / NT;  // vectors per thread per chunk
   constexpr int CHUNK = m2_chunk_bytes(NS, NP, NCC, JT);
   extern __shared__ __align__(16) char m2_smem[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, h = lane >> 5, li = lane & 31;
   const int js = bx_ % jsplit;
   const int64_t ib = bx_ / jsplit;
-  const int64_t I0 = (ib * 4 + w) * (32 * ITW);
+  const int64_t I0 = (ib * NW + w) * (32 * ITW);
   const bool active = I0 < n_rows;
   (void)ka;
   typename G::template Operand<NS> B[ITW];
@@ -214,11 +215,11 @@ __device__ __forceinline__ void mv_multi2_kernel_body(
   const uint4* xsrc[3] = {reinterpret_cast<const uint4*>(xh), reinterpret_cast<const uint4*>(xl),
                           reinterpret_cast<const uint4*>(xm)};
   uint4 pre[VPT];
-  auto fetch = [&](int64_t t) {
+  auto fetch = [&](int64_t t, uint4* dst) __attribute__((always_inline)) {
     const int nt = (int)((t1 - t) < JT ? (t1 - t) : JT);
 #pragma unroll
     for (int u2 = 0; u2 < VPT; ++u2) {
-      const int vi = u2 * 256 + threadIdx.x;
+      const int vi = u2 * NT + threadIdx.x;
       uint4 v = {0u, 0u, 0u, 0u};
       if (vi < VA) {
         if (vi < nt * 32 * vpr) v = reinterpret_cast<const uint4*>(gsrc + t * 32 * gb)[vi];
@@ -228,13 +229,13 @@ __device__ __forceinline__ void mv_multi2_kernel_body(
         const int pp = rem / (NCC * 2 * 64), off = rem % (NCC * 2 * 64);  // off = (q*2+u)*64 + lane
         if (tt < nt) v = xsrc[pp][(t + tt) * (NCC * 128) + off];
       }
-      pre[u2] = v;
+      dst[u2] = v;
     }
   };
-  auto stash = [&](char* dst) {
+  auto stash = [&](char* dst) __attribute__((always_inline)) {
 #pragma unroll
     for (int u2 = 0; u2 < VPT; ++u2) {
-      const int vi = u2 * 256 + threadIdx.x;
+      const int vi = u2 * NT + threadIdx.x;
       if (vi < VA) {
         *reinterpret_cast<uint4*>(dst + (vi / vpr) * lb + (vi % vpr) * 16) = pre[u2];
       } else if (vi < VA + VX) {
@@ -248,13 +249,13 @@ __device__ __forceinline__ void mv_multi2_kernel_body(
 #pragma unroll
     for (int q = 0; q < NCC; ++q) acc[it][q] = floatx16{0};
 
-  fetch(t0);
+  fetch(t0, pre);
   stash(m2_smem);
   __syncthreads();
   bool odd = false;
   for (int64_t tc = t0; tc < t1; tc += JT) {
     const bool more = tc + JT < t1;
-    if (more) fetch(tc + JT);
+    if (more) fetch(tc + JT, pre);
     const int nt = (int)((t1 - tc) < JT ? (t1 - tc) : JT);
     if (active) {
       const char* cur = m2_smem + (odd ? CHUNK : 0);
@@ -265,6 +266,53 @@ __device__ __forceinline__ void mv_multi2_kernel_body(
 #pragma unroll
         for (int q = 0; q < NS; ++q)
           A.v[q] = __builtin_bit_cast(typename G::vec, *reinterpret_cast<const shortx8*>(rowp + q * 32 + h * 16));
+        if constexpr (GX == GX_PLXH && ITW == 1) {
+          // BF16-exact X, exp tile split hi/lo: 4 NCC products per tile.  The
+          // X operands of the first K half are read before the Gram, the exps
+          // and conversions of the second half run in the shadow of the first
+          // half's MFMAs, and the second half's X operands are read meanwhile,
+          // so neither the VALU work nor the LDS latency stalls the MFMA pipe.
+          // Per accumulator the products keep the order lo.X, hi.X of K half
+          // 0, then of K half 1.
+          bf16x8 X0[NCC], X1[NCC];
+#pragma unroll
+          for (int q = 0; q < NCC; ++q) X0[q] = __builtin_bit_cast(bf16x8, xs[tt * XT + (q * 2) * 64 + lane]);
+          __builtin_amdgcn_sched_barrier(0);
+          const floatx16 D = G::template gram_exact<NS>(A, B[0]);
+          bf16x8 Ph[2], Pl[2];
+          auto split = [&](int u) __attribute__((always_inline)) {
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+              const float pv = kexp<CLAMP, false>(D[8 * u + v], clampv);
+              const uint16_t hb = f32_to_bf16_bits(pv);
+              Ph[u][v] = __builtin_bit_cast(__bf16, hb);
+              Pl[u][v] = __builtin_bit_cast(__bf16, f32_to_bf16_bits(pv - bf16_bits_to_f32(hb)));
+            }
+          };
+          split(0);
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int q = 0; q < NCC; ++q) {
+            acc[0][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Pl[0], X0[q], acc[0][q], 0, 0, 0);
+            acc[0][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ph[0], X0[q], acc[0][q], 0, 0, 0);
+          }
+#pragma unroll
+          for (int q = 0; q < NCC; ++q) X1[q] = __builtin_bit_cast(bf16x8, xs[tt * XT + (q * 2 + 1) * 64 + lane]);
+          split(1);
+#pragma unroll
+          for (int q = 0; q < 2 * NCC; ++q) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+            __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // two VALU
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // one LDS read
+          }
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int q = 0; q < NCC; ++q) {
+            acc[0][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Pl[1], X1[q], acc[0][q], 0, 0, 0);
+            acc[0][q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ph[1], X1[q], acc[0][q], 0, 0, 0);
+          }
+          continue;
+        }
         bf16x8 Xh[NCC][2], Xl[NCC][2], Xm[NCC][2];
 #pragma unroll
         for (int q = 0; q < NCC; ++q)
@@ -339,13 +387,15 @@ __device__ __forceinline__ void mv_multi2_kernel_body(
       }
 }
 
-template <int GM, int NS, int GX, int NCC, int JT, int ITW, bool CLAMP>
-__global__ __launch_bounds__(256) void mv_multi2_kernel(
+template <int GM, int NS, int GX, int NCC, int JT, int ITW, bool CLAMP, int NW = 4>
+__global__ __launch_bounds__(64 * NW) void mv_multi2_kernel(
     const void* __restrict__ colpack, const void* __restrict__ rowpack, int ka, int64_t n_rows,
     int64_t n_cols, const bf16x8* __restrict__ xh, const bf16x8* __restrict__ xl,
     const bf16x8* __restrict__ xm, int kcols, int c_base, float* __restrict__ part, int64_t ldp,
     int jsplit, float clampv) {
-  mv_multi2_kernel_body<GM, NS, GX, NCC, JT, ITW, CLAMP>(colpack, rowpack, ka, n_rows, n_cols, xh, xl, xm, kcols, c_base, part, ldp, jsplit, clampv, (int)blockIdx.x, (int)blockIdx.y);
+  mv_multi2_kernel_body<GM, NS, GX, NCC, JT, ITW, CLAMP, NW>(colpack, rowpack, ka, n_rows, n_cols, xh, xl, xm,
+                                                             kcols, c_base, part, ldp, jsplit, clampv,
+                                                             (int)blockIdx.x, (int)blockIdx.y);
 }
 
 static int launch_multi(int mode, int gx, const void* colpack, const void* rowpack, int ka,
@@ -448,7 +498,6 @@ int product_multi_f32(Operator& op, int precision, const Side& rows, const Side&
     bf16x8* xl = nparts >= 2 ? reinterpret_cast<bf16x8*>(op.xpack.as<char>() + part_bytes) : nullptr;
     bf16x8* xm = nparts >= 3 ? reinterpret_cast<bf16x8*>(op.xpack.as<char>() + 2 * part_bytes) : nullptr;
     const int jsplit2 = jsplit;
-    const dim3 block(256);
     for (int c0 = 0; c0 < k;) {
       // column chunks of 32 per pass: the 3-way split keeps 2 for its register budget;
       // hi/lo and BF16 take 4 when more than 64 columns remain (half the Gram passes);
@@ -464,26 +513,29 @@ int product_multi_f32(Operator& op, int precision, const Side& rows, const Side&
     constexpr size_t lds2 = 2 * (size_t)m2_chunk_bytes(NS, M2Parts<GXM>::value, NC, JTV);            \
     /* one row tile per wave for the register-heavy 3-way split and 4-chunk passes */                \
     constexpr int ITWV = (GXM == GX_SPLIT3 || NC >= 4) ? 1 : 2;                                       \
-    dim3 grid2((unsigned)(ceil_div(n_rows, 128 * ITWV) * jsplit2));                                   \
+    /* 8-chunk passes: 8-wave blocks, so each block streams the packed X once for 256 rows */        \
+    constexpr int NWV = NC >= 8 ? 8 : 4;                                                              \
+    dim3 grid2((unsigned)(ceil_div(n_rows, 32 * NWV * ITWV) * jsplit2)), block2(64 * NWV);            \
     if (f16s) {                                                                                       \
       if (clamp)                                                                                      \
-        hipLaunchKernelGGL((mv_multi2_kernel<GRAM_F16S, NS, GXM, NC, JTV, ITWV, true>), grid2, block,  \
-                           lds2, st, op.self.col16h.ptr, op.self.row16h.ptr, op.ka16h, n_rows, n_cols, \
-                           (const bf16x8*)xh, (const bf16x8*)xl, (const bf16x8*)xm, k, c0, part, kp,   \
-                           jsplit2, clampv);                                                          \
+        hipLaunchKernelGGL((mv_multi2_kernel<GRAM_F16S, NS, GXM, NC, JTV, ITWV, true, NWV>), grid2,    \
+                           block2, lds2, st, op.self.col16h.ptr, op.self.row16h.ptr, op.ka16h, n_rows, \
+                           n_cols, (const bf16x8*)xh, (const bf16x8*)xl, (const bf16x8*)xm, k, c0,    \
+                           part, kp, jsplit2, clampv);                                                \
       else                                                                                            \
-        hipLaunchKernelGGL((mv_multi2_kernel<GRAM_F16S, NS, GXM, NC, JTV, ITWV, false>), grid2, block, \
-                           lds2, st, op.self.col16h.ptr, op.self.row16h.ptr, op.ka16h, n_rows, n_cols, \
-                           (const bf16x8*)xh, (const bf16x8*)xl, (const bf16x8*)xm, k, c0, part, kp,   \
-                           jsplit2, clampv);                                                          \
+        hipLaunchKernelGGL((mv_multi2_kernel<GRAM_F16S, NS, GXM, NC, JTV, ITWV, false, NWV>), grid2,   \
+                           block2, lds2, st, op.self.col16h.ptr, op.self.row16h.ptr, op.ka16h, n_rows, \
+                           n_cols, (const bf16x8*)xh, (const bf16x8*)xl, (const bf16x8*)xm, k, c0,    \
+                           part, kp, jsplit2, clampv);                                                \
     } else if (clamp)                                                                                 \
-      hipLaunchKernelGGL((mv_multi2_kernel<LAKER_PREC_MIXED, NS, GXM, NC, JTV, ITWV, true>), grid2,    \
-                         block, lds2, st, colpack, rowpack, ka, n_rows, n_cols, (const bf16x8*)xh,    \
+      hipLaunchKernelGGL((mv_multi2_kernel<LAKER_PREC_MIXED, NS, GXM, NC, JTV, ITWV, true, NWV>), grid2, \
+                         block2, lds2, st, colpack, rowpack, ka, n_rows, n_cols, (const bf16x8*)xh,   \
                          (const bf16x8*)xl, (const bf16x8*)xm, k, c0, part, kp, jsplit2, clampv);     \
     else                                                                                              \
-      hipLaunchKernelGGL((mv_multi2_kernel<LAKER_PREC_MIXED, NS, GXM, NC, JTV, ITWV, false>), grid2,   \
-                         block, lds2, st, colpack, rowpack, ka, n_rows, n_cols, (const bf16x8*)xh,    \
-                         (const bf16x8*)xl, (const bf16x8*)xm, k, c0, part, kp, jsplit2, clampv);     \
+      hipLaunchKernelGGL((mv_multi2_kernel<LAKER_PREC_MIXED, NS, GXM, NC, JTV, ITWV, false, NWV>),     \
+                         grid2, block2, lds2, st, colpack, rowpack, ka, n_rows, n_cols,               \
+                         (const bf16x8*)xh, (const bf16x8*)xl, (const bf16x8*)xm, k, c0, part, kp,    \
+                         jsplit2, clampv);                                                            \
   } while (0)
 #define LAKER_M2C(NS, GXM, NC)                                                                        \
   do {                                                                                                \
