@@ -230,17 +230,24 @@ __device__ inline bool block_alpha1(const double* __restrict__ part, int nblk, c
 // k = 1, alpha folded in (the column-dot partials of p.Ap, p.p, Ap.Ap in
 // dots[0 .. 3 nblk)): x += alpha p ; r -= alpha Ap, and rr[blk] = the block's
 // sum of r_new^2.  rr must not overlap dots (other blocks still read them).
+// Block 0 keeps this iteration's r.z in rz[1] (the next beta's divisor) and,
+// with bump_it, counts the iteration as completed (the fused k = 1 loop: the
+// residual test that follows reads the count, no kernel after it writes it).
 template <class T>
 __global__ __launch_bounds__(256) void pcg_update_xr_rr_alpha_kernel(T* __restrict__ x, T* __restrict__ r,
                                                                      const T* __restrict__ p,
                                                                      const T* __restrict__ ap,
                                                                      const double* __restrict__ dots, int nblk,
-                                                                     const double* __restrict__ rz, int64_t n,
+                                                                     double* __restrict__ rz, int64_t n,
                                                                      double* __restrict__ rr,
-                                                                     PcgState* __restrict__ st) {
+                                                                     PcgState* __restrict__ st, int bump_it) {
   if (st->done) return;  // written by an earlier kernel: uniform over the grid
   double a;
   if (!block_alpha1(dots, nblk, rz, st, &a)) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    rz[1] = rz[0];  // r.z of this iteration, for the next beta
+    if (bump_it) st->it += 1;
+  }
   __shared__ double red[16];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double r2 = 0.0;
@@ -283,19 +290,12 @@ __global__ void pcg_resid_kernel(const double* __restrict__ part, int nblk, int 
   pcg_resid_kernel_body(part, nblk, k, st, (int)blockIdx.x, (int)blockIdx.y);
 }
 
-// k = 1: the bookkeeping of pcg_resid_kernel (from the ||r||^2 partials rr of
-// the update kernel; every block evaluates the same test, block 0 records it)
-// followed, unless the run just stopped, by the r.z column-dot partials of
-// coldots_kernel<T, 1> (blocks of rpb rows).  The preconditioner kernels that
-// run between the update and this kernel do not read the state.  The
-// iteration counter moves on here only when the run stops (every block must
-// read the same st->it); otherwise pcg_beta_kernel advances it.
-template <class T>
-__global__ __launch_bounds__(256) void pcg_resid_rz_kernel(const double* __restrict__ rr, int nrr,
-                                                           const T* __restrict__ r, const T* __restrict__ z,
-                                                           int64_t n, int64_t rpb, double* __restrict__ part,
-                                                           PcgState* __restrict__ st) {
-  if (st->done) return;  // written by an earlier kernel: uniform over the grid
+// The residual test of the fused k = 1 loop, from the ||r||^2 partials rr of
+// the update kernel (which has already counted the iteration, and no kernel
+// of this iteration after it changes the count): every block evaluates it
+// identically, block 0 records it.  Returns true (uniformly over the block)
+// when the run stops here.
+__device__ inline bool block_resid_test1(const double* __restrict__ rr, int nrr, PcgState* __restrict__ st) {
   __shared__ int stop;
   const int lane = threadIdx.x & 63;
   if (threadIdx.x < 64) {
@@ -304,23 +304,35 @@ __global__ __launch_bounds__(256) void pcg_resid_rz_kernel(const double* __restr
       const int it = st->it, max_iter = st->max_iter;
       const double res = sqrt(s);
       const bool conv = res / st->rhs_norm <= st->tol;
-      const bool last = !conv && it + 1 >= max_iter;
+      const bool last = !conv && it >= max_iter;
       stop = (conv || last) ? 1 : 0;
       if (blockIdx.x == 0) {
         st->res_norm = res;
         if (conv) {
           st->done = PCG_CONVERGED;
-          st->iterations = it + 1;
+          st->iterations = it;
         } else if (last) {
           st->done = PCG_MAXITER;
           st->iterations = max_iter;
         }
-        if (conv || last) st->it = it + 1;
       }
     }
   }
   __syncthreads();
-  if (stop) return;
+  return stop != 0;
+}
+
+// k = 1: the residual test (block_resid_test1) followed, unless the run just
+// stopped, by the r.z column-dot partials of coldots_kernel<T, 1> (blocks of
+// rpb rows).  The preconditioner kernels that run between the update and this
+// kernel do not read the state.
+template <class T>
+__global__ __launch_bounds__(256) void pcg_resid_rz_kernel(const double* __restrict__ rr, int nrr,
+                                                           const T* __restrict__ r, const T* __restrict__ z,
+                                                           int64_t n, int64_t rpb, double* __restrict__ part,
+                                                           PcgState* __restrict__ st) {
+  if (st->done) return;  // written by an earlier kernel: uniform over the grid
+  if (block_resid_test1(rr, nrr, st)) return;
   const int64_t r0 = (int64_t)blockIdx.x * rpb;
   const int64_t r1 = (r0 + rpb < n) ? r0 + rpb : n;
   double acc = 0.0;
@@ -368,6 +380,31 @@ __global__ void pcg_update_p_kernel(T* __restrict__ p, const T* __restrict__ z,
                                     const double* __restrict__ beta, int64_t n, int k,
                                     const PcgState* __restrict__ st) {
   pcg_update_p_kernel_body<T>(p, z, beta, n, k, st, (int)blockIdx.x, (int)blockIdx.y);
+}
+
+// k = 1: beta = r.z / rz_old and p = z + beta p in one kernel.  Every block
+// sums the r.z partials in the order of pcg_beta_kernel and divides by
+// rz_old = rz[1] (saved by the alpha-update kernel before rz[0] moves on), so
+// beta and p are bit-identical to the two-kernel form; block 0 stores the new
+// rz[0] (no other block reads it).
+template <class T>
+__global__ __launch_bounds__(256) void pcg_beta_update_p1_kernel(T* __restrict__ p, const T* __restrict__ z,
+                                                                 const double* __restrict__ part, int nblk,
+                                                                 double* __restrict__ rz, int64_t n,
+                                                                 PcgState* __restrict__ st) {
+  if (st->done) return;  // written by an earlier kernel: uniform over the grid
+  __shared__ double bsh;
+  if (threadIdx.x < 64) {
+    const double v = wave_sum_part(part, 0, nblk, 1, 0, threadIdx.x);
+    if (threadIdx.x == 0) {
+      bsh = v / rz[1];
+      if (blockIdx.x == 0) rz[0] = v;
+    }
+  }
+  __syncthreads();
+  const double beta = bsh;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (T)((double)z[i] + beta * (double)p[i]);
 }
 
 // set-up scalars: rz = colsum(r z), rhs_norm = ||b||_F.  One block.
@@ -616,7 +653,8 @@ static int pcg_run(Operator& op, Precond* pc, const T* B, int k, T* X, const T* 
       // alpha and the ||r||^2 partials come out of the update itself
       // (restarted runs redo the partials after the residual replacement in tail)
       hipLaunchKernelGGL((pcg_update_xr_rr_alpha_kernel<T>), eg, eb, 0, st, b.x, b.r, (const T*)b.p,
-                         (const T*)b.ap, (const double*)b.part, nblk, (const double*)b.rz, n, rr_part, b.st);
+                         (const T*)b.ap, (const double*)b.part, nblk, b.rz, n, rr_part, b.st,
+                         restart_freq <= 0 ? 1 : 0);
       LAKER_KCHECK("pcg_update_xr_rr_alpha_kernel");
     } else {
       hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.alpha, b.st);
@@ -644,8 +682,13 @@ static int pcg_run(Operator& op, Precond* pc, const T* B, int k, T* X, const T* 
     } else {
       LAKER_TRY(coldots<T>(b, 1, b.r, b.z, nullptr, nullptr, nullptr, nullptr, n, k, true, st));
     }
-    hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.beta, b.st,
-                       fuse_resid ? 1 : 0);
+    if (fuse_resid) {
+      hipLaunchKernelGGL((pcg_beta_update_p1_kernel<T>), eg, eb, 0, st, b.p, (const T*)b.z, (const double*)b.part,
+                         nblk, b.rz, n, b.st);
+      LAKER_KCHECK("pcg_beta_update_p1_kernel");
+      return LAKER_OK;
+    }
+    hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.beta, b.st, 0);
     LAKER_KCHECK("pcg_beta_kernel");
     hipLaunchKernelGGL((pcg_update_p_kernel<T>), eg, eb, 0, st, b.p, b.z, b.beta, n, k, b.st);
     LAKER_KCHECK("pcg_update_p_kernel");
@@ -759,12 +802,16 @@ __global__ __launch_bounds__(256) void pcg_update_x64_rr_alpha_kernel(double* __
                                                                       const float* __restrict__ p,
                                                                       const float* __restrict__ ap,
                                                                       const double* __restrict__ dots, int nblk,
-                                                                      const double* __restrict__ rz, int64_t n,
+                                                                      double* __restrict__ rz, int64_t n,
                                                                       double* __restrict__ rr,
                                                                       PcgState* __restrict__ st) {
   if (st->done) return;  // written by an earlier kernel: uniform over the grid
   double a;
   if (!block_alpha1(dots, nblk, rz, st, &a)) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    rz[1] = rz[0];  // r.z of this iteration, for the next beta
+    st->it += 1;    // this iteration is complete
+  }
   __shared__ double red[16];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double r2 = 0.0;
@@ -1284,7 +1331,7 @@ int laker_solve_mixed(laker_operator_t oh, laker_precond_t ph, const double* B, 
     if (!dots_done) LAKER_TRY(coldots<float>(b, 3, b.p, b.ap, b.p, b.p, b.ap, b.ap, n, k, true, st));
     if (k == 1) {
       hipLaunchKernelGGL(pcg_update_x64_rr_alpha_kernel, eg, eb, 0, st, X, b.r, (const float*)b.p,
-                         (const float*)b.ap, (const double*)b.part, nblk, (const double*)b.rz, n, rr_part, b.st);
+                         (const float*)b.ap, (const double*)b.part, nblk, b.rz, n, rr_part, b.st);
       LAKER_KCHECK("pcg_update_x64_rr_alpha_kernel");
     } else {
       hipLaunchKernelGGL(pcg_alpha_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.alpha, b.st);
@@ -1303,11 +1350,13 @@ int laker_solve_mixed(laker_operator_t oh, laker_precond_t ph, const double* B, 
       hipLaunchKernelGGL((pcg_resid_rz_kernel<float>), dim3(nblk), dim3(256), 0, st, (const double*)rr_part,
                          (int)eg.x, (const float*)b.r, (const float*)b.z, n, b.rpb, b.part, b.st);
       LAKER_KCHECK("pcg_resid_rz_kernel");
-    } else {
-      LAKER_TRY(coldots<float>(b, 1, b.r, b.z, nullptr, nullptr, nullptr, nullptr, n, k, true, st));
+      hipLaunchKernelGGL((pcg_beta_update_p1_kernel<float>), eg, eb, 0, st, b.p, (const float*)b.z,
+                         (const double*)b.part, nblk, b.rz, n, b.st);
+      LAKER_KCHECK("pcg_beta_update_p1_kernel");
+      return LAKER_OK;
     }
-    hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.beta, b.st,
-                       k == 1 ? 1 : 0);
+    LAKER_TRY(coldots<float>(b, 1, b.r, b.z, nullptr, nullptr, nullptr, nullptr, n, k, true, st));
+    hipLaunchKernelGGL(pcg_beta_kernel, dim3(1), dim3(256), 0, st, b.part, nblk, k, b.rz, b.beta, b.st, 0);
     LAKER_KCHECK("pcg_beta_kernel");
     hipLaunchKernelGGL((pcg_update_p_kernel<float>), eg, eb, 0, st, b.p, (const float*)b.z,
                        (const double*)b.beta, n, k, (const PcgState*)b.st);
