@@ -202,73 +202,138 @@ constexpr int PNB = 64;
 
 // Unblocked Cholesky of the nb x nb diagonal block at (k0,k0) and the inverse
 // of its factor into dinv (64 x 64 row-major, identity-padded beyond nb), by
-// one wave: lane i holds row i in registers (the loops over columns are
-// unrolled, so every register index is fixed) and nothing branches on data.
-// Right-looking: at step k every lane scales its entry of column k by
-// 1 / sqrt(pivot), the column goes through LDS, and each lane applies
-// l_ik l_jk to its entries j > k, so every entry receives its updates in the
-// order k = 0, 1, ...  Sets *info = global index + 1 of the first
-// non-positive pivot (replaced by 1e300 so the factorisation runs on
-// without NaNs).
+// four waves: lane i of wave w holds row i, columns w, w + 4, ..., w + 60 of
+// the block in registers (the steps are unrolled at compile time, so every
+// register index is fixed) and nothing branches on data.  Right-looking: at
+// step k the wave owning column k takes the pivot from lane k, scales its
+// entries of the column by 1 / sqrt(pivot) and publishes the column in LDS;
+// after one barrier every wave applies l_ik l_jk to its entries j > k, so
+// every entry receives its updates in the order k = 0, 1, ...  With a
+// quarter of a row per lane, the column values a step needs are read from
+// LDS together and the barrier is the only wait.  Sets *info = global index
+// + 1 of the first non-positive pivot (replaced by 1e300 so the factorisation
+// runs on without NaNs).
 __device__ __forceinline__ double lane_bcast(double v, int src) {
   const int2 w = __builtin_bit_cast(int2, v);
   return __builtin_bit_cast(double, int2{__builtin_amdgcn_readlane(w.x, src), __builtin_amdgcn_readlane(w.y, src)});
 }
 
-__global__ __launch_bounds__(64) void potrf_diag_kernel(double* __restrict__ A, int64_t lda, int64_t k0, int nb,
-                                                        int* __restrict__ info, double* __restrict__ dinv) {
-  __shared__ double col[2][PNB];             // column k of L, double-buffered across steps
-  __shared__ double Lc[PNB * (PNB + 1)];     // L column-major: Lc[c (PNB + 1) + r] = L[r][c]
-  const int i = threadIdx.x;
-  double a[PNB];
+constexpr int PW = PNB / 4;  // columns (factorisation) or rows (inverse) per wave
+
+struct PotrfShared {
+  double col[2][PNB];          // column k of L, double-buffered across steps
+  double xb[2][PNB];           // row p of the inverse, double-buffered across steps
+  double Lc[PNB * (PNB + 1)];  // L column-major: Lc[c (PNB + 1) + r] = L[r][c]
+  int bad_at;                  // first non-positive pivot (local index) or -1
+};
+
+// Step K of the factorisation for wave w (columns 4 c + w).
+template <int K>
+__device__ __forceinline__ void potrf_fact_step(double (&a)[PW], int i, int w, int nb, PotrfShared& sh) {
+  if ((K & 3) == w) {  // uniform over the wave
+    constexpr int kl = K >> 2;
+    double piv = lane_bcast(a[kl], K);
+    const bool bad = !(piv > 0.0);
+    if (bad && K < nb && i == 0 && sh.bad_at < 0) sh.bad_at = K;
+    piv = bad ? 1e300 : piv;
+    const double d = sqrt(piv);
+    const double r = 1.0 / d;
+    a[kl] = i == K ? d : (i > K ? a[kl] * r : 0.0);
+    sh.col[K & 1][i] = a[kl];
+    sh.Lc[K * (PNB + 1) + i] = a[kl];
+  }
+  __syncthreads();
+  constexpr int c1 = K >> 2;  // columns 4 c + w > K: all c > c1, and c = c1 when w > K & 3
+  if constexpr (c1 < PW) {
+    const double lik = sh.col[K & 1][i];
+    double lj[PW];
+#pragma unroll
+    for (int c = c1; c < PW; ++c) lj[c] = sh.col[K & 1][4 * c + w];
+    if (w > (K & 3)) a[c1] -= lik * lj[c1];
+#pragma unroll
+    for (int c = c1 + 1; c < PW; ++c) a[c] -= lik * lj[c];
+  }
+}
+
+// Step P of the inverse for wave w (rows 4 c + w of column i).
+template <int P>
+__device__ __forceinline__ void potrf_inv_step(double (&v)[PW], int i, int w, double* __restrict__ dinv,
+                                               PotrfShared& sh) {
+  const double* lp = sh.Lc + P * (PNB + 1);
+  if ((P & 3) == w) {
+    const double xp = v[P >> 2] / lp[P];
+    dinv[P * PNB + i] = xp;
+    sh.xb[P & 1][i] = xp;
+  }
+  __syncthreads();
+  constexpr int c1 = P >> 2;
+  if constexpr (c1 < PW) {
+    const double xp = sh.xb[P & 1][i];
+    double lq[PW];
+#pragma unroll
+    for (int c = c1; c < PW; ++c) lq[c] = lp[4 * c + w];
+    if (w > (P & 3)) v[c1] -= lq[c1] * xp;
+#pragma unroll
+    for (int c = c1 + 1; c < PW; ++c) v[c] -= lq[c] * xp;
+  }
+}
+
+template <int K>
+struct PotrfSteps {
+  static __device__ __forceinline__ void fact(double (&a)[PW], int i, int w, int nb, PotrfShared& sh) {
+    potrf_fact_step<K>(a, i, w, nb, sh);
+    PotrfSteps<K + 1>::fact(a, i, w, nb, sh);
+  }
+  static __device__ __forceinline__ void inv(double (&v)[PW], int i, int w, double* __restrict__ dinv,
+                                             PotrfShared& sh) {
+    potrf_inv_step<K>(v, i, w, dinv, sh);
+    PotrfSteps<K + 1>::inv(v, i, w, dinv, sh);
+  }
+};
+template <>
+struct PotrfSteps<PNB> {
+  static __device__ __forceinline__ void fact(double (&)[PW], int, int, int, PotrfShared&) {}
+  static __device__ __forceinline__ void inv(double (&)[PW], int, int, double* __restrict__, PotrfShared&) {}
+};
+
+__global__ __launch_bounds__(256) void potrf_diag_kernel(double* __restrict__ A, int64_t lda, int64_t k0, int nb,
+                                                         int* __restrict__ info, double* __restrict__ dinv) {
+  __shared__ PotrfShared sh;
+  const int i = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (threadIdx.x == 0) sh.bad_at = -1;
+  double a[PW];
   {
     const bool rin = i < nb;
     const double* row = A + (k0 + (rin ? i : 0)) * lda + k0;
 #pragma unroll
-    for (int c = 0; c < PNB; ++c) {
-      const double g = row[c < nb ? c : 0];
-      a[c] = (rin && c < nb) ? g : (i == c ? 1.0 : 0.0);
+    for (int c = 0; c < PW; ++c) {
+      const int cc = 4 * c + w;
+      const double g = row[cc < nb ? cc : 0];
+      a[c] = (rin && cc < nb) ? g : (i == cc ? 1.0 : 0.0);
     }
   }
-  int firstbad = -1;  // uniform: the pivot is read from lane k
-#pragma unroll
-  for (int k = 0; k < PNB; ++k) {
-    double piv = lane_bcast(a[k], k);
-    const bool bad = !(piv > 0.0);
-    if (bad && firstbad < 0 && k < nb) firstbad = k;
-    piv = bad ? 1e300 : piv;
-    const double d = sqrt(piv);
-    const double r = 1.0 / d;
-    a[k] = i == k ? d : (i > k ? a[k] * r : 0.0);
-    col[k & 1][i] = a[k];
-    Lc[k * (PNB + 1) + i] = a[k];
-    __syncthreads();
-#pragma unroll
-    for (int j = k + 1; j < PNB; ++j) a[j] -= a[k] * col[k & 1][j];
-  }
-  if (i == 0 && firstbad >= 0 && *info == 0) *info = (int)(k0 + firstbad + 1);
+  __syncthreads();  // bad_at initialised
+  PotrfSteps<0>::fact(a, i, w, nb, sh);
   if (i < nb) {
     double* row = A + (k0 + i) * lda + k0;
 #pragma unroll
-    for (int c = 0; c < PNB; ++c)
-      if (c < nb) row[c] = c <= i ? a[c] : 0.0;
+    for (int c = 0; c < PW; ++c) {
+      const int cc = 4 * c + w;
+      if (cc < nb) row[cc] = cc <= i ? a[c] : 0.0;
+    }
   }
   // column t = i of L^{-1} by forward substitution, column-oriented: once x_p
   // is known, every later row takes v_q -= L[q][p] x_p, so each v_q receives
   // its terms in the order p = 0, 1, ... of the row-oriented dot product.
   // v_p stays exactly zero for p < t (zero right-hand side, zero x so far),
-  // so x_p = v_p / L[p][p] is zero there without a test.
-  double v[PNB];
+  // so x_p = v_p / L[p][p] is zero there without a test.  Wave w holds the
+  // rows 4 c + w of the column.
+  double v[PW];
 #pragma unroll
-  for (int q = 0; q < PNB; ++q) v[q] = q == i ? 1.0 : 0.0;
-#pragma unroll
-  for (int p = 0; p < PNB; ++p) {
-    const double* lp = Lc + p * (PNB + 1);
-    const double xp = v[p] / lp[p];
-    dinv[p * PNB + i] = xp;
-#pragma unroll
-    for (int q = p + 1; q < PNB; ++q) v[q] -= lp[q] * xp;
-  }
+  for (int c = 0; c < PW; ++c) v[c] = 4 * c + w == i ? 1.0 : 0.0;
+  PotrfSteps<0>::inv(v, i, w, dinv, sh);
+  __syncthreads();
+  if (threadIdx.x == 0 && sh.bad_at >= 0 && *info == 0) *info = (int)(k0 + sh.bad_at + 1);
 }
 
 // dst[r][c] = src[r][c] for r < rows, c < cols
@@ -307,7 +372,7 @@ int potrf_lower(double* A, int64_t lda, int64_t n, int* info_dev, DevBuf* ws, Tr
   for (int64_t k0 = 0; k0 < n; k0 += PNB) {
     const int nb = (int)std::min<int64_t>(PNB, n - k0);
     double* dk = dinv + (k0 / PNB) * PNB * PNB;
-    hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(64), 0, st, A, lda, k0, nb, info_dev, dk);
+    hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, st, A, lda, k0, nb, info_dev, dk);
     LAKER_KCHECK("potrf_diag_kernel");
     const int64_t rem = n - k0 - nb;
     if (rem > 0) {
