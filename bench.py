@@ -240,38 +240,26 @@ def main():
         op_f.close()
         pre_f.close()
 
-    # dominant kernel: the k=1 mixed matvec of the inner PCG; time with HIP events on the stream.
-    # In PCG every product is followed by HBM-bound work (its reduction, the vector updates, the
-    # preconditioner apply), so each product is timed by its own event pair with a preconditioner
-    # apply between two products, as in the solve; the same products back to back (a sustained
-    # VALU load at which the clock drops) are reported beside it.
+    # dominant kernel: the k=1 mixed matvec of the inner PCG; time with HIP events on the stream
+    # around 20 products back to back (avg_launch_ms, the figure the roofline uses)
     x1 = torch.randn(n, device=dev)
     reps = 20
-    mv, mv_b2b = {}, {}
+    mv = {}
     y1 = torch.empty_like(x1)
     for prec, code in (("mixed", N.PREC_MIXED), ("fp32", N.PREC_FP32)):
         opx = laker.AttentionKernelOperator(e, lambda_reg=args.lambda_reg, precision=prec)
         # straight C-ABI calls: the launches stay ahead of the GPU, so the events
-        # bracket the kernels (product + reduction) and no host gaps
+        # bracket the kernels and no host gaps
         h, xp, yp, sp = opx._h.h, N.ptr(x1), N.ptr(y1), N.stream_ptr()
         N.check(N.lib.laker_matvec(h, xp, 1, 1, yp, 1, code, sp))
-        pre_m.apply(x1)
         torch.cuda.synchronize()
-        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
-        for a, b in evs:
-            a.record(stream)
-            N.lib.laker_matvec(h, xp, 1, 1, yp, 1, code, sp)
-            b.record(stream)
-            pre_m.apply(x1)
-        torch.cuda.synchronize()
-        mv[prec] = sum(a.elapsed_time(b) for a, b in evs) / reps
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         ev0.record(stream)
         for _ in range(reps):
             N.lib.laker_matvec(h, xp, 1, 1, yp, 1, code, sp)
         ev1.record(stream)
         ev1.synchronize()
-        mv_b2b[prec] = ev0.elapsed_time(ev1) / reps
+        mv[prec] = ev0.elapsed_time(ev1) / reps
         opx.close()
     # preconditioner apply (HBM-bound): two passes over Q (n x N_r fp32)
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -309,9 +297,7 @@ def main():
         "algorithmic_flops_per_launch": flops_mv,
         "algorithmic_bytes_per_launch": bytes_mv,
         "avg_launch_ms": mv["mixed"],
-        "timing": ("HIP events around each of 20 laker_matvec calls on the launch stream, one "
-                   "preconditioner apply between calls as in PCG"),
-        "back_to_back_ms": mv_b2b["mixed"],
+        "timing": "HIP events on the launch stream around 20 back-to-back laker_matvec calls (kernel + reduction)",
         "binding_resource": "VALU issue (v_exp_f32 plus the row and column FMAs of every exp); MFMA busy "
                             "19 %, HBM under 5 %: see profiles/counters_mv_sym1_n100000_d10.txt",
         "mfma": {"achieved_TFLOPs": flops_mv / t_mv / 1e12, "peak_TFLOPs": MFMA_BF16_PEAK_TFLOPS,
@@ -457,6 +443,18 @@ def main():
         t_c4 = cpu_c4_step_time(O, laker, 2000, 500, steps=3)
         print(f"[bench] CPU C4 step (2000+500): {t_c4:.2f} s", file=sys.stderr, flush=True)
 
+        # C3: sequential plain CPU solves of the serving stream, bounded: the requests among ids
+        # 0..99 with n <= 2500 (the oracle's fp32 fit: CCCP build + PCG on all k right-hand sides,
+        # lambda 1e-2, tol 1e-6), one after another
+        from laker.serving import make_request
+
+        c3_reqs = [q for q in (make_request(r) for r in range(100)) if q.n <= 2500]
+        t0 = time.perf_counter()
+        for q in c3_reqs:
+            O.fit(q.x, q.y, embedding_dim=q.embedding_dim, lambda_reg=q.lambda_reg, pcg_tol=args.tol, pcg_max_iter=1000)
+        t_c3 = time.perf_counter() - t0
+        print(f"[bench] CPU C3: {len(c3_reqs)} sequential solves in {t_c3:.1f} s", file=sys.stderr, flush=True)
+
         cpu = {"value": 1.0 / t_solve_cpu, "unit": "solves/sec", "cores": torch.get_num_threads(),
                "kind": "port",
                "sample": (f"oracle (the reference's tiling, fp32) on {cpu_model}: one full k=1 product at "
@@ -467,6 +465,12 @@ def main():
                "c2_iteration_s": t_c2_iter,
                "c2_note": "one 512-column product at n=50,000, d=16 (the batched PCG iteration's dominant cost); "
                           "a solve takes about 380 such iterations (extrapolation)",
+               "c3_sequential": {"requests": len(c3_reqs), "solves_per_sec": len(c3_reqs) / t_c3,
+                                 "elapsed_s": t_c3,
+                                 "n_range": [min(q.n for q in c3_reqs), max(q.n for q in c3_reqs)],
+                                 "rhs_total": sum(q.k for q in c3_reqs),
+                                 "sample": "the requests among ids 0..99 with n <= 2500, solved one after "
+                                           "another by the oracle's fp32 fit (timed, not extrapolated)"},
                "c4_step_s_reduced": t_c4,
                "c4_note": "one training step at 2,000 train + 500 validation points; the products scale as n^2, "
                           f"so 20,000 + 5,000 points take about {100 * t_c4:.0f} s per step (extrapolation)"}

@@ -257,40 +257,62 @@ mv_sym1_kernel(const void* __restrict__ colpack, const void* __restrict__ rowpac
 }
 
 // y[i] = lam xr[i] (+ y[i]) + sum_seg part_r[seg][i] + sum_{b < i/SYM_RB} part_c[coff(b) + i - SYM_RB (b+1)]
-// dots (optional, block = 256 rows): dots[q * gridDim.x + block] = the
-// block's sum of x.y (q = 0), x.x (q = 1) and y.y (q = 2) over its rows, with
-// x = Xr and y the stored Y, in the order and with the block reduction of the
-// PCG's column-dot kernel for 256-row blocks (bit-identical partials).
-__global__ void mv_sym_reduce_kernel(const SymPart* __restrict__ part_r, const SymPart* __restrict__ part_c,
-                                     int64_t n, double lam, const float* __restrict__ Xr, int64_t ldxr,
-                                     float* __restrict__ Y, int64_t ldy, int accumulate,
-                                     const int* __restrict__ skip, double* __restrict__ dots) {
+// Block = 256 rows x 4 threads per row (1024 threads).  The partials of row i
+// are summed in eight chains u = 0..7 (chain u takes the partials 8 m + u of
+// the row list, then of the column list) plus one tail chain for what is
+// left after the last full group of eight; thread c of the row runs chains c
+// and c + 4, thread 3 the tail as well, and thread 0 adds the nine chain sums
+// in a fixed tree.  Spreading the chains over four threads keeps four times
+// more loads in flight than one thread per row would, at the same sum order.
+// dots (optional): dots[q * gridDim.x + block] = the block's sum of x.y
+// (q = 0), x.x (q = 1) and y.y (q = 2) over its 256 rows, with x = Xr and y
+// the stored Y, in the order and with the block reduction of the PCG's
+// column-dot kernel for 256-row blocks (bit-identical partials).
+constexpr int SRED_ROWS = 256;
+constexpr int SRED_TPR = 4;  // threads per row
+
+__global__ __launch_bounds__(SRED_ROWS * SRED_TPR) void mv_sym_reduce_kernel(
+    const SymPart* __restrict__ part_r, const SymPart* __restrict__ part_c, int64_t n, double lam,
+    const float* __restrict__ Xr, int64_t ldxr, float* __restrict__ Y, int64_t ldy, int accumulate,
+    const int* __restrict__ skip, double* __restrict__ dots) {
   if (skip && *skip) return;  // uniform over the grid
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double xd = 0.0, yd = 0.0;
+  __shared__ double csum[9][SRED_ROWS];
+  const int r = threadIdx.x & (SRED_ROWS - 1), c = threadIdx.x / SRED_ROWS;
+  const int64_t i = (int64_t)blockIdx.x * SRED_ROWS + r;
   if (i < n) {
     const int64_t ntiles = (n + 31) / 32;
     const int64_t b = i / SYM_RB;
     const int64_t nseg = (ntiles + SYM_SEG - 1) / SYM_SEG;
-    // eight independent chains (fixed assignment) keep the loads in flight
-    double a8[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    double tail = 0.0;
-    int64_t sg = (SYM_DT * b) / SYM_SEG;  // first segment of row block b
-    for (; sg + 8 <= nseg; sg += 8) {
-#pragma unroll
-      for (int u = 0; u < 8; ++u) a8[u] += part_r[(sg + u) * n + i];
+    const int64_t sg0 = (SYM_DT * b) / SYM_SEG;  // first segment of row block b
+    const int64_t gr = (nseg - sg0) / 8, gc = b / 8;  // full groups of eight
+    double a0 = 0.0, a1 = 0.0;
+    const SymPart* pr = part_r + (sg0 + c) * n + i;
+#pragma unroll 4
+    for (int64_t m = 0; m < gr; ++m) {
+      a0 += pr[(8 * m) * n];
+      a1 += pr[(8 * m + 4) * n];
     }
-    for (; sg < nseg; ++sg) tail += part_r[sg * n + i];
-    int64_t bb = 0;
-    for (; bb + 8 <= b; bb += 8) {
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int64_t q = bb + u;
-        a8[u] += part_c[sym_coff(q, n) + i - SYM_RB * (q + 1)];
-      }
+#pragma unroll 4
+    for (int64_t m = 0; m < gc; ++m) {
+      const int64_t q0 = 8 * m + c, q1 = q0 + 4;
+      a0 += part_c[sym_coff(q0, n) + i - SYM_RB * (q0 + 1)];
+      a1 += part_c[sym_coff(q1, n) + i - SYM_RB * (q1 + 1)];
     }
-    for (; bb < b; ++bb) tail += part_c[sym_coff(bb, n) + i - SYM_RB * (bb + 1)];
-    double s = (((a8[0] + a8[1]) + (a8[2] + a8[3])) + ((a8[4] + a8[5]) + (a8[6] + a8[7]))) + tail;
+    csum[c][r] = a0;
+    csum[c + 4][r] = a1;
+    if (c == SRED_TPR - 1) {
+      double tail = 0.0;
+      for (int64_t sg = sg0 + 8 * gr; sg < nseg; ++sg) tail += part_r[sg * n + i];
+      for (int64_t bb = 8 * gc; bb < b; ++bb) tail += part_c[sym_coff(bb, n) + i - SYM_RB * (bb + 1)];
+      csum[8][r] = tail;
+    }
+  }
+  __syncthreads();
+  double xd = 0.0, yd = 0.0;
+  if (c == 0 && i < n) {
+    double s = (((csum[0][r] + csum[1][r]) + (csum[2][r] + csum[3][r])) +
+                ((csum[4][r] + csum[5][r]) + (csum[6][r] + csum[7][r]))) +
+               csum[8][r];
     if (Xr) s += lam * (double)Xr[i * ldxr];
     if (accumulate) s += (double)Y[i * ldy];
     const float y = (float)s;
@@ -300,13 +322,20 @@ __global__ void mv_sym_reduce_kernel(const SymPart* __restrict__ part_r, const S
       yd = (double)y;
     }
   }
-  if (dots) {  // uniform: every thread of the block takes part
-    __shared__ double red[16];
+  if (dots) {  // uniform; the block sum of the first 256 threads, as block_sum forms it for 4 waves
+    __shared__ double red[4];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const double v[3] = {xd * yd, xd * xd, yd * yd};
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      const double t = block_sum(v[q], red);
-      if (threadIdx.x == 0) dots[(int64_t)q * gridDim.x + blockIdx.x] = t;
+      const double w = wave_sum(v[q]);
+      __syncthreads();
+      if (lane == 0 && wid < 4) red[wid] = w;
+      __syncthreads();
+      if (wid == 0) {
+        const double t = wave_sum(lane < 4 ? red[lane] : 0.0);
+        if (lane == 0) dots[(int64_t)q * gridDim.x + blockIdx.x] = t;
+      }
     }
   }
 }
@@ -373,7 +402,7 @@ int launch_sym1(Operator& op, int mode, const void* colpack, const void* rowpack
   }
 #undef LAKER_S1
   LAKER_KCHECK("mv_sym1_kernel");
-  hipLaunchKernelGGL(mv_sym_reduce_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st,
+  hipLaunchKernelGGL(mv_sym_reduce_kernel, dim3((unsigned)ceil_div(n, SRED_ROWS)), dim3(SRED_ROWS * SRED_TPR), 0, st,
                      (const SymPart*)part_r, (const SymPart*)part_c, n, lam, Xr, ldxr, Y, ldy, accumulate, skip,
                      Xr ? dots : nullptr);
   LAKER_KCHECK("mv_sym_reduce_kernel");
