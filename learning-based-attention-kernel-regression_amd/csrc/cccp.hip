@@ -67,30 +67,64 @@ __global__ void randn_kernel(T* __restrict__ out, int64_t count, uint64_t seed, 
   out[i] = bf16_exact ? (T)bf16_bits_to_f32(f32_to_bf16_bits((float)v)) : v;
 }
 
-// column sums of squares of an n x m row-major matrix -> out[m] (FP64), grid = m
+// Column sums of squares of an n x m row-major matrix -> out[m] (FP64).  Sum
+// order per column c: 256 strided chains (chain t takes rows t, t + 256,
+// t + 512, ... in order), then block_sum over the 256 chain sums.  Pass 1
+// (colsumsq_chains_kernel) runs the chains with 64 consecutive columns per
+// wave, so every load of a wave is one contiguous row segment; pass 2
+// (colsumsq_final_kernel, one block per column) forms the block sum.
+constexpr int CSQ_CHAINS = 256;
+
 template <class T>
-__global__ void colsumsq_kernel(const T* __restrict__ U, int64_t n, int m, double* __restrict__ out) {
-  __shared__ double sm[16];
-  const int c = blockIdx.x;
+__global__ __launch_bounds__(256) void colsumsq_chains_kernel(const T* __restrict__ U, int64_t n, int m,
+                                                              double* __restrict__ chains) {
+  const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int t = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (c >= m) return;
   double s = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+#pragma unroll 8
+  for (int64_t i = t; i < n; i += CSQ_CHAINS) {
     const double v = (double)U[i * m + c];
     s += v * v;
   }
-  s = block_sum(s, sm);
+  chains[(int64_t)t * m + c] = s;
+}
+
+__global__ __launch_bounds__(CSQ_CHAINS) void colsumsq_final_kernel(const double* __restrict__ chains, int m,
+                                                                    double* __restrict__ out) {
+  __shared__ double sm[16];
+  const int c = blockIdx.x;
+  double s = block_sum(chains[(int64_t)threadIdx.x * m + c], sm);
   if (threadIdx.x == 0) out[c] = s;
 }
 
-// U[:, c] /= max(sqrt(ss[c]), eps)  (in U's dtype, like the reference)
+template <class T>
+static int colsumsq(const T* U, int64_t n, int m, double* out, DevBuf& chains, hipStream_t st) {
+  LAKER_TRY(chains.ensure((size_t)CSQ_CHAINS * m * sizeof(double), st));
+  hipLaunchKernelGGL((colsumsq_chains_kernel<T>), dim3((unsigned)ceil_div(m, 64), CSQ_CHAINS / 4), dim3(256), 0, st,
+                     U, n, m, chains.as<double>());
+  LAKER_KCHECK("colsumsq_chains_kernel");
+  hipLaunchKernelGGL(colsumsq_final_kernel, dim3((unsigned)m), dim3(CSQ_CHAINS), 0, st,
+                     (const double*)chains.as<double>(), m, out);
+  LAKER_KCHECK("colsumsq_final_kernel");
+  return LAKER_OK;
+}
+
+// U[:, c] /= max(sqrt(ss[c]), eps) in U's dtype (like the reference), and
+// V = (double)U when V is given.  grid.y = row blocks of 64 rows
 template <class T>
 __global__ void colscale_kernel(T* __restrict__ U, int64_t n, int m, const double* __restrict__ ss,
-                                double eps) {
-  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n * m) return;
-  const int c = (int)(idx % m);
+                                double eps, double* __restrict__ V) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m) return;
   const T nrm = (T)sqrt(ss[c]);
   const T den = nrm > (T)eps ? nrm : (T)eps;
-  U[idx] = U[idx] / den;
+  const int64_t i0 = (int64_t)blockIdx.y * 64, i1 = i0 + 64 < n ? i0 + 64 : n;
+  for (int64_t i = i0; i < i1; ++i) {
+    const T u = U[i * m + c] / den;
+    U[i * m + c] = u;
+    if (V) V[i * m + c] = (double)u;
+  }
 }
 
 template <class TI, class TO>
@@ -1158,7 +1192,9 @@ int precond_apply_half(Precond& pc, const float* X, float* Z, hipStream_t st) {
 // non-zero *info (device); the caller reads it.
 static int cholqr_pass(const double* Vin, double* Vout, int64_t n, int nr, double shift, double* G, double* Linv,
                        double* Racc, double* tmp, int* info, DevBuf* ws, TriWork& tw, hipStream_t st) {
-  LAKER_TRY((gemm<double, double, double>(true, false, nr, nr, n, 1.0, Vin, nr, Vin, nr, 0.0, G, nr, ws, st)));
+  // lower triangle only: the Cholesky reads no other part of G
+  LAKER_TRY((gemm<double, double, double>(true, false, nr, nr, n, 1.0, Vin, nr, Vin, nr, 0.0, G, nr, ws, st,
+                                           GEMM_LOWER)));
   if (shift > 0.0) {
     hipLaunchKernelGGL(add_diag_kernel, dim3((unsigned)ceil_div(nr, 256)), dim3(256), 0, st, G, (int64_t)nr,
                        shift);
@@ -1169,9 +1205,9 @@ static int cholqr_pass(const double* Vin, double* Vout, int64_t n, int nr, doubl
   // Linv = L^{-1}
   LAKER_TRY(set_identity(Linv, nr, 1.0, st));
   LAKER_TRY(trsm_lower_left(G, nr, nr, Linv, nr, nr, ws, tw, st));
-  // Vout = Vin Linv^T
+  // Vout = Vin Linv^T  (Linv^T upper triangular: column j needs k <= j)
   LAKER_TRY((gemm<double, double, double>(false, true, n, nr, nr, 1.0, Vin, nr, Linv, nr, 0.0, Vout, nr,
-                                           nullptr, st)));
+                                           nullptr, st, GEMM_BTRI)));
   // Racc <- L^T Racc   (L^T upper = R factor of this pass)
   LAKER_TRY((gemm<double, double, double>(true, false, nr, nr, nr, 1.0, G, nr, Racc, nr, 0.0, tmp, nr,
                                            nullptr, st)));
@@ -1188,7 +1224,7 @@ static int cholqr_pass(const double* Vin, double* Vout, int64_t n, int nr, doubl
 // end (a failed first pass makes the second one's output meaningless, and the
 // shifted route then starts over from src).
 static int cholqr_block(const void* src, int dtype, int64_t n, int nr, double* V, double* Racc, DevBuf* ws,
-                        hipStream_t st) {
+                        hipStream_t st, bool v_loaded = false) {
   const int64_t nn = (int64_t)nr * nr;
   DevBuf Vtmp, G, Linv, tmp, infob;
   TriWork tri;
@@ -1201,6 +1237,10 @@ static int cholqr_block(const void* src, int dtype, int64_t n, int nr, double* V
   int* info = infob.as<int>();
   double* Vt = Vtmp.as<double>();
   auto load_v = [&]() -> int {
+    if (v_loaded) {  // V already holds the FP64 copy of src (first attempt only)
+      v_loaded = false;
+      return set_identity(Racc, nr, 1.0, st);
+    }
     if (dtype == LAKER_F32)
       hipLaunchKernelGGL((convert_kernel<float, double>), dim3((unsigned)ceil_div(n * nr, 256)), dim3(256), 0,
                          st, (const float*)src, V, n * nr);
@@ -1245,30 +1285,29 @@ static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, i
   const size_t esz = pc->dtype == LAKER_F32 ? 4 : 8;
   const int64_t nn = (int64_t)nr * nr;
   (void)op;
-  // 2. column normalisation in the solver dtype
-  DevBuf ss;
+  // 2. column normalisation in the solver dtype; the scaling pass also writes
+  // the FP64 copy the Cholesky-QR starts from
+  DevBuf ss, chains, V, Racc;
   LAKER_TRY(ss.ensure((size_t)nr * 8, st));
+  LAKER_TRY(V.ensure((size_t)n * nr * 8, st));
+  LAKER_TRY(Racc.ensure(nn * 8, st));
+  const dim3 sg((unsigned)ceil_div(nr, 256), (unsigned)ceil_div(n, 64)), sb(256);
   if (pc->dtype == LAKER_F32) {
-    hipLaunchKernelGGL((colsumsq_kernel<float>), dim3(nr), dim3(256), 0, st, U.as<float>(), n, nr, ss.as<double>());
-    LAKER_KCHECK("colsumsq_kernel");
-    hipLaunchKernelGGL((colscale_kernel<float>), dim3((unsigned)ceil_div(n * nr, 256)), dim3(256), 0, st,
-                       U.as<float>(), n, nr, ss.as<double>(), epsilon);
+    LAKER_TRY(colsumsq(U.as<float>(), n, nr, ss.as<double>(), chains, st));
+    hipLaunchKernelGGL((colscale_kernel<float>), sg, sb, 0, st, U.as<float>(), n, nr, (const double*)ss.as<double>(),
+                       epsilon, V.as<double>());
     LAKER_KCHECK("colscale_kernel");
   } else {
-    hipLaunchKernelGGL((colsumsq_kernel<double>), dim3(nr), dim3(256), 0, st, U.as<double>(), n, nr, ss.as<double>());
-    LAKER_KCHECK("colsumsq_kernel");
-    hipLaunchKernelGGL((colscale_kernel<double>), dim3((unsigned)ceil_div(n * nr, 256)), dim3(256), 0, st,
-                       U.as<double>(), n, nr, ss.as<double>(), epsilon);
+    LAKER_TRY(colsumsq(U.as<double>(), n, nr, ss.as<double>(), chains, st));
+    hipLaunchKernelGGL((colscale_kernel<double>), sg, sb, 0, st, U.as<double>(), n, nr,
+                       (const double*)ss.as<double>(), epsilon, V.as<double>());
     LAKER_KCHECK("colscale_kernel");
   }
   LAKER_HIP_CHECK(hipGetLastError());
 
   // 3. Cholesky-QR2 in FP64 (shifted Cholesky-QR3 on failure)
-  DevBuf V, Racc;
-  LAKER_TRY(V.ensure((size_t)n * nr * 8, st));
-  LAKER_TRY(Racc.ensure(nn * 8, st));
   {
-    const int rc = cholqr_block(U.ptr, pc->dtype, n, nr, V.as<double>(), Racc.as<double>(), &pc->ws, st);
+    const int rc = cholqr_block(U.ptr, pc->dtype, n, nr, V.as<double>(), Racc.as<double>(), &pc->ws, st, true);
     if (rc == LAKER_ENUMERIC) return fail(LAKER_ENUMERIC, "CCCP build: Cholesky-QR of the probe responses failed");
     LAKER_TRY(rc);
   }
@@ -1663,11 +1702,12 @@ int laker_orthonormalize(void* X, int64_t n, int32_t k, int32_t dtype, hipStream
   else
     LAKER_HIP_CHECK(hipMemcpyAsync(W.ptr, X, (size_t)nk * 8, hipMemcpyDeviceToDevice, st));
   LAKER_KCHECK("convert_kernel");
-  hipLaunchKernelGGL((colsumsq_kernel<double>), dim3(k), dim3(256), 0, st, W.as<double>(), n, k, ss.as<double>());
-  LAKER_KCHECK("colsumsq_kernel");
-  hipLaunchKernelGGL((colscale_kernel<double>), eg, eb, 0, st, W.as<double>(), n, k, ss.as<double>(), 1e-300);
+  DevBuf chains;
+  LAKER_TRY(colsumsq(W.as<double>(), n, k, ss.as<double>(), chains, st));
+  hipLaunchKernelGGL((colscale_kernel<double>), dim3((unsigned)ceil_div(k, 256), (unsigned)ceil_div(n, 64)), eb, 0, st,
+                     W.as<double>(), n, k, (const double*)ss.as<double>(), 1e-300, V.as<double>());
   LAKER_KCHECK("colscale_kernel");
-  const int rc = cholqr_block(W.ptr, LAKER_F64, n, k, V.as<double>(), Racc.as<double>(), &ws, st);
+  const int rc = cholqr_block(W.ptr, LAKER_F64, n, k, V.as<double>(), Racc.as<double>(), &ws, st, true);
   if (rc == LAKER_ENUMERIC) return fail(LAKER_ENUMERIC, "orthonormalize: the block is numerically rank-deficient");
   LAKER_TRY(rc);
   if (dtype == LAKER_F32)

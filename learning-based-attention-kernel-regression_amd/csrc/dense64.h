@@ -4,22 +4,27 @@
 
 namespace laker {
 
+// Shapes of gemm's operands and output:
+//   GEMM_FULL   every output tile
+//   GEMM_LOWER  only the 64 x 64 output tiles on or below the diagonal are
+//               formed (the lower triangle of a symmetric product); with
+//               beta == 0 the tiles above are written as zeros, else untouched
+//   GEMM_BTRI   op(B)[k][n] == 0 for k > n (op(B) upper triangular): each
+//               output tile stops its K loop after its last column
+constexpr int GEMM_FULL = 0, GEMM_LOWER = 1, GEMM_BTRI = 2;
+
 // C = alpha * op(A) * op(B) + beta * C, row-major, FP64 accumulation.
 // op(A) is M x K, op(B) is K x N.  ws (optional) enables split-K.
 template <class TA, class TB, class TC>
 int gemm(bool tra, bool trb, int64_t M, int64_t N, int64_t K, double alpha, const TA* A,
          int64_t lda, const TB* B, int64_t ldb, double beta, TC* C, int64_t ldc, DevBuf* ws,
-         hipStream_t st);
+         hipStream_t st, int kmode = GEMM_FULL);
 
 // Work space of potrf_lower / trsm_lower_left: the inverses of the 64 x 64
-// diagonal blocks of the last factor (dinv) and a panel buffer.
+// diagonal blocks of the last factor (dinv).
 struct TriWork {
   DevBuf dinv;
-  DevBuf panel;
-  void bind(hipStream_t st) {
-    dinv.bind(st);
-    panel.bind(st);
-  }
+  void bind(hipStream_t st) { dinv.bind(st); }
 };
 
 // In-place lower Cholesky of the n x n matrix A (upper triangle zeroed).

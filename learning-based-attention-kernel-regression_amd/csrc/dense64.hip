@@ -87,12 +87,26 @@ __device__ __forceinline__ void gemm_kernel_body(int64_t M, int64_t N, int64_t K
                                                    const TA* __restrict__ A, int64_t lda,
                                                    const TB* __restrict__ B, int64_t ldb,
                                                    double beta, TC* __restrict__ C, int64_t ldc,
-                                                   double* __restrict__ part, int64_t kchunk, int bx_, int by_, int bz_) {
+                                                   double* __restrict__ part, int64_t kchunk, int bx_, int by_, int bz_,
+                                                   int kmode = GEMM_FULL) {
   __shared__ double As[GBK][GBM + 1];
   __shared__ double Bs[GBK][GBN + 1];
   const int64_t m0 = (int64_t)by_ * GBM, n0 = (int64_t)bx_ * GBN;
   const int64_t k0 = (int64_t)bz_ * kchunk;
-  const int64_t k1 = (k0 + kchunk < K) ? k0 + kchunk : K;
+  int64_t k1 = (k0 + kchunk < K) ? k0 + kchunk : K;
+  if (kmode == GEMM_LOWER && n0 > m0) {
+    // a tile strictly above the diagonal: not formed; zeros when C is overwritten
+    // directly (with split-K the reduction writes them)
+    if (!part && beta == 0.0) {
+      for (int e = threadIdx.x; e < GBM * GBN; e += blockDim.x) {
+        const int64_t gm = m0 + e / GBN, gn = n0 + e % GBN;
+        if (gm < M && gn < N) C[gm * ldc + gn] = (TC)0.0;
+      }
+    }
+    return;
+  }
+  // op(B)[k][n] = 0 for k > n: the tile's columns need k < n0 + GBN only
+  if (kmode == GEMM_BTRI && n0 + GBN < k1) k1 = n0 + GBN;
   doublex4 acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -125,16 +139,21 @@ __global__ __launch_bounds__(256) void gemm_kernel(int64_t M, int64_t N, int64_t
                                                    const TA* __restrict__ A, int64_t lda,
                                                    const TB* __restrict__ B, int64_t ldb,
                                                    double beta, TC* __restrict__ C, int64_t ldc,
-                                                   double* __restrict__ part, int64_t kchunk) {
-  gemm_kernel_body<TA, TB, TC, TRA, TRB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, part, kchunk, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
+                                                   double* __restrict__ part, int64_t kchunk, int kmode) {
+  gemm_kernel_body<TA, TB, TC, TRA, TRB>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, part, kchunk, (int)blockIdx.x,
+                                         (int)blockIdx.y, (int)blockIdx.z, kmode);
 }
 
 template <class TC>
 __global__ void gemm_splitk_reduce(const double* __restrict__ part, int64_t M, int64_t N, int ks,
-                                   double alpha, double beta, TC* __restrict__ C, int64_t ldc) {
+                                   double alpha, double beta, TC* __restrict__ C, int64_t ldc, int kmode) {
   int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= M * N) return;
   const int64_t m = idx / N, n = idx % N;
+  if (kmode == GEMM_LOWER && n / GBN > m / GBM) {  // tile not formed
+    if (beta == 0.0) C[m * ldc + n] = (TC)0.0;
+    return;
+  }
   double s = 0.0;
   for (int z = 0; z < ks; ++z) s += part[(int64_t)z * M * N + idx];
   double v = alpha * s;
@@ -145,7 +164,7 @@ __global__ void gemm_splitk_reduce(const double* __restrict__ part, int64_t M, i
 template <class TA, class TB, class TC>
 int gemm(bool tra, bool trb, int64_t M, int64_t N, int64_t K, double alpha, const TA* A,
          int64_t lda, const TB* B, int64_t ldb, double beta, TC* C, int64_t ldc, DevBuf* ws,
-         hipStream_t st) {
+         hipStream_t st, int kmode) {
   if (M <= 0 || N <= 0) return LAKER_OK;
   const int64_t tiles = ceil_div(M, GBM) * ceil_div(N, GBN);
   int ks = 1;
@@ -162,7 +181,7 @@ int gemm(bool tra, bool trb, int64_t M, int64_t N, int64_t K, double alpha, cons
   dim3 grid((unsigned)ceil_div(N, GBN), (unsigned)ceil_div(M, GBM), (unsigned)ks), block(256);
 #define LAKER_GEMM(TRA_, TRB_)                                                                  \
   hipLaunchKernelGGL((gemm_kernel<TA, TB, TC, TRA_, TRB_>), grid, block, 0, st, M, N, K, alpha, A, \
-                     lda, B, ldb, beta, C, ldc, part, kchunk)
+                     lda, B, ldb, beta, C, ldc, part, kchunk, kmode)
   if (tra) {
     if (trb) LAKER_GEMM(true, true); else LAKER_GEMM(true, false);
   } else {
@@ -172,7 +191,7 @@ int gemm(bool tra, bool trb, int64_t M, int64_t N, int64_t K, double alpha, cons
   LAKER_HIP_CHECK(hipGetLastError());
   if (ks > 1) {
     hipLaunchKernelGGL((gemm_splitk_reduce<TC>), dim3((unsigned)ceil_div(M * N, 256)), dim3(256), 0,
-                       st, part, M, N, ks, alpha, beta, C, ldc);
+                       st, part, M, N, ks, alpha, beta, C, ldc, kmode);
     LAKER_KCHECK("gemm_splitk_reduce");
     LAKER_HIP_CHECK(hipGetLastError());
   }
@@ -182,7 +201,7 @@ int gemm(bool tra, bool trb, int64_t M, int64_t N, int64_t K, double alpha, cons
 #define LAKER_INST_GEMM(TA, TB, TC)                                                             \
   template int gemm<TA, TB, TC>(bool, bool, int64_t, int64_t, int64_t, double, const TA*,     \
                                 int64_t, const TB*, int64_t, double, TC*, int64_t, DevBuf*,  \
-                                hipStream_t);
+                                hipStream_t, int);
 LAKER_INST_GEMM(double, double, double)
 LAKER_INST_GEMM(float, float, double)
 LAKER_INST_GEMM(float, double, double)
@@ -336,16 +355,6 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double* __restrict__ A,
   if (threadIdx.x == 0 && sh.bad_at >= 0 && *info == 0) *info = (int)(k0 + sh.bad_at + 1);
 }
 
-// dst[r][c] = src[r][c] for r < rows, c < cols
-__global__ void copy2d_kernel(const double* __restrict__ src, int64_t lds_, double* __restrict__ dst,
-                              int64_t ldd, int64_t rows, int cols) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= rows * cols) return;
-  const int64_t r = idx / cols;
-  const int c = (int)(idx % cols);
-  dst[r * ldd + c] = src[r * lds_ + c];
-}
-
 __global__ void zero_upper_kernel(double* __restrict__ A, int64_t lda, int64_t n) {
   int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n * n) return;
@@ -353,22 +362,11 @@ __global__ void zero_upper_kernel(double* __restrict__ A, int64_t lda, int64_t n
   if (j > i) A[i * lda + j] = 0.0;
 }
 
-static int copy2d(const double* src, int64_t lds_, double* dst, int64_t ldd, int64_t rows, int cols,
-                  hipStream_t st) {
-  if (rows <= 0 || cols <= 0) return LAKER_OK;
-  hipLaunchKernelGGL(copy2d_kernel, dim3((unsigned)ceil_div(rows * cols, 256)), dim3(256), 0, st, src,
-                     lds_, dst, ldd, rows, cols);
-  LAKER_KCHECK("copy2d_kernel");
-  return LAKER_OK;
-}
-
 int potrf_lower(double* A, int64_t lda, int64_t n, int* info_dev, DevBuf* ws, TriWork& tw,
                 hipStream_t st) {
   const int64_t nblk = ceil_div(n, PNB);
   LAKER_TRY(tw.dinv.ensure((size_t)nblk * PNB * PNB * sizeof(double)));
-  LAKER_TRY(tw.panel.ensure((size_t)std::max<int64_t>(n, 1) * PNB * sizeof(double)));
   double* dinv = tw.dinv.as<double>();
-  double* panel = tw.panel.as<double>();
   for (int64_t k0 = 0; k0 < n; k0 += PNB) {
     const int nb = (int)std::min<int64_t>(PNB, n - k0);
     double* dk = dinv + (k0 / PNB) * PNB * PNB;
@@ -376,15 +374,17 @@ int potrf_lower(double* A, int64_t lda, int64_t n, int* info_dev, DevBuf* ws, Tr
     LAKER_KCHECK("potrf_diag_kernel");
     const int64_t rem = n - k0 - nb;
     if (rem > 0) {
-      // L21 = A21 L11^{-T} into the panel buffer (rem x nb), then back into A
+      // L21 = A21 L11^{-T} in place: the product has one column tile (nb <= 64)
+      // and K = nb is too short for split-K, so each workgroup reads its rows
+      // of A21 in full before it writes them
       double* A21 = A + (k0 + nb) * lda + k0;
       LAKER_TRY((gemm<double, double, double>(false, true, rem, nb, nb, 1.0, A21, lda, dk, PNB, 0.0,
-                                               panel, nb, ws, st)));
-      LAKER_TRY(copy2d(panel, nb, A21, lda, rem, nb, st));
-      // A22 -= L21 L21^T  (full square of the trailing block; upper part is scratch)
+                                               A21, lda, ws, st)));
+      // A22 -= L21 L21^T on the tiles at or below the diagonal (the factor
+      // never reads the upper triangle; it is zeroed at the end)
       double* A22 = A + (k0 + nb) * lda + k0 + nb;
-      LAKER_TRY((gemm<double, double, double>(false, true, rem, rem, nb, -1.0, panel, nb, panel, nb,
-                                               1.0, A22, lda, ws, st)));
+      LAKER_TRY((gemm<double, double, double>(false, true, rem, rem, nb, -1.0, A21, lda, A21, lda,
+                                               1.0, A22, lda, ws, st, GEMM_LOWER)));
     }
   }
   hipLaunchKernelGGL(zero_upper_kernel, dim3((unsigned)ceil_div(n * n, 256)), dim3(256), 0, st, A,
@@ -402,21 +402,20 @@ int trsm_lower_left(const double* L, int64_t ldl, int64_t n, double* B, int64_t 
   const int64_t nblk = ceil_div(n, PNB);
   if (tw.dinv.bytes < (size_t)nblk * PNB * PNB * sizeof(double))
     return fail(LAKER_ESTATE, "trsm_lower_left: no diagonal-block inverses (call potrf_lower first)");
-  LAKER_TRY(tw.panel.ensure((size_t)PNB * std::max<int64_t>(m, 1) * sizeof(double)));
   const double* dinv = tw.dinv.as<double>();
-  double* panel = tw.panel.as<double>();
   for (int64_t k0 = 0; k0 < n; k0 += PNB) {
     const int nb = (int)std::min<int64_t>(PNB, n - k0);
     const double* dk = dinv + (k0 / PNB) * PNB * PNB;
-    // B_k <- L_kk^{-1} B_k
+    // B_k <- L_kk^{-1} B_k in place: one row tile (nb <= 64) and K = nb (no
+    // split-K), so each workgroup reads its columns of B_k in full before it
+    // writes them
     LAKER_TRY((gemm<double, double, double>(false, false, nb, m, nb, 1.0, dk, PNB, B + k0 * ldb, ldb,
-                                             0.0, panel, m, ws, st)));
-    LAKER_TRY(copy2d(panel, m, B + k0 * ldb, ldb, nb, (int)m, st));
+                                             0.0, B + k0 * ldb, ldb, ws, st)));
     const int64_t rem = n - k0 - nb;
     if (rem > 0) {
       // B[k0+nb:, :] -= L[k0+nb:, k0:k0+nb] * B_k
       LAKER_TRY((gemm<double, double, double>(false, false, rem, m, nb, -1.0,
-                                               L + (k0 + nb) * ldl + k0, ldl, panel, m, 1.0,
+                                               L + (k0 + nb) * ldl + k0, ldl, B + k0 * ldb, ldb, 1.0,
                                                B + (k0 + nb) * ldb, ldb, ws, st)));
     }
   }

@@ -180,12 +180,32 @@ __device__ __forceinline__ void mv_reduce_kernel_body(const PT* __restrict__ par
                                  const YT* __restrict__ B, YT* __restrict__ Y, int64_t ldy,
                                  int accumulate, const int* __restrict__ skip, int bx_, int by_) {
   if (skip && *skip) return;
-  int64_t idx = (int64_t)bx_ * blockDim.x + threadIdx.x;
-  if (idx >= n_rows * k) return;
-  const int64_t i = idx / k;
-  const int c = (int)(idx % k);
+  const int64_t idx = (int64_t)bx_ * blockDim.x + threadIdx.x;
+  const int64_t tot = n_rows * k;
+  if (idx >= tot) return;
+  int64_t i;
+  int c;
+  if (tot <= 0x7fffffff) {  // 32-bit index arithmetic
+    const uint32_t u = (uint32_t)idx, q = u / (uint32_t)k;
+    i = q;
+    c = (int)(u - q * (uint32_t)k);
+  } else {
+    i = idx / k;
+    c = (int)(idx % k);
+  }
+  // sum over the splits in order, eight loads in flight
+  const PT* p = part + i * ldp + c;
+  const int64_t pstride = n_rows * ldp;
   double s = 0.0;
-  for (int js = 0; js < jsplit; ++js) s += (double)part[((int64_t)js * n_rows + i) * ldp + c];
+  int js = 0;
+  for (; js + 8 <= jsplit; js += 8) {
+    PT v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = p[(int64_t)(js + u) * pstride];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += (double)v[u];
+  }
+  for (; js < jsplit; ++js) s += (double)p[(int64_t)js * pstride];
   if (Xr) s += lam * (double)Xr[i * ldx + c];
   if constexpr (RES) {
     Y[i * ldy + c] = (YT)((double)B[i * ldy + c] - s);
