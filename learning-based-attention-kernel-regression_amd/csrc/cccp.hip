@@ -1355,13 +1355,17 @@ static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, i
   int it_done = 0;
   const double n_over_nr = (double)n / (double)nr;
   for (int it = 0; it < max_iter; ++it) {
-    hipLaunchKernelGGL(form_m_kernel, dim3((unsigned)ceil_div(nn, 256)), dim3(256), 0, st, Cb.as<double>(),
-                       Mb.as<double>(), (int64_t)nr, a);
-    LAKER_KCHECK("form_m_kernel");
     LAKER_HIP_CHECK(hipMemsetAsync(infob.ptr, 0, sizeof(int), st));
-    LAKER_TRY(potrf_lower(Mb.as<double>(), nr, nr, infob.as<int>(), &pc->ws, tri, st));
     LAKER_HIP_CHECK(hipMemcpyAsync(Zb.ptr, Rq, nn * 8, hipMemcpyDeviceToDevice, st));
-    LAKER_TRY(trsm_lower_left(Mb.as<double>(), nr, nr, Zb.as<double>(), nr, nr, &pc->ws, tri, st));
+    if (it > 0) {
+      // Z = L^{-1} R with L L^T = M = aI + C.  The first iteration starts from
+      // C = 0, a = 1: M = I, whose factor and solve return R unchanged, bit for bit
+      hipLaunchKernelGGL(form_m_kernel, dim3((unsigned)ceil_div(nn, 256)), dim3(256), 0, st, Cb.as<double>(),
+                         Mb.as<double>(), (int64_t)nr, a);
+      LAKER_KCHECK("form_m_kernel");
+      LAKER_TRY(potrf_lower(Mb.as<double>(), nr, nr, infob.as<int>(), &pc->ws, tri, st));
+      LAKER_TRY(trsm_lower_left(Mb.as<double>(), nr, nr, Zb.as<double>(), nr, nr, &pc->ws, tri, st));
+    }
     hipLaunchKernelGGL(cccp_weights_kernel, dim3(nr), dim3(256), 0, st, Zb.as<double>(), Rq, Rw.as<double>(),
                        nr, n_over_nr, epsilon);
     LAKER_KCHECK("cccp_weights_kernel");
@@ -1417,7 +1421,7 @@ static int build_from_u(Operator& op, std::unique_ptr<Precond>& pc, DevBuf& U, i
   LAKER_TRY(partb.ensure((size_t)std::max(ns_blocks, upd_blocks) * 2 * 8, st));
   hipLaunchKernelGGL(ns_init_kernel, dim3(1), dim3(1), 0, st, nss);
   LAKER_KCHECK("ns_init_kernel");
-  constexpr int NS_POLL = 4;
+  constexpr int NS_POLL = 8;
   const double ns_tol = 1e-13 * std::sqrt((double)nr);
   NsState hns{};
   for (int it = 0; it < 200; ++it) {

@@ -224,7 +224,9 @@ class PreconditionedConjugateGradient:
         b = rhs.to(device=compute_device(), dtype=torch.float64).contiguous()
         b2 = b.unsqueeze(1) if b.dim() == 1 else b
         tol = self.tol
-        bnorm = float(torch.linalg.norm(b2))
+        # ||b|| on the host only where it is used (warm start, log line): it
+        # costs a device synchronisation before the solve starts
+        bnorm = float(torch.linalg.norm(b2)) if (x0 is not None or self.verbose) else None
         if x0 is not None:
             # warm start: solve for the correction A d = b - A x0; the stop stays
             # relative to ||b|| (reference laker/solvers.py:148-166)
