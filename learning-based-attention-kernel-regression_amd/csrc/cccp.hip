@@ -63,8 +63,9 @@ __global__ void randn_kernel(T* __restrict__ out, int64_t count, uint64_t seed, 
   const uint64_t b = splitmix64(seed * 0x632BE59BD9B4E019ull + 2 * (uint64_t)i + 1);
   const double u1 = ((a >> 11) + 1.0) * (1.0 / 9007199254740993.0);
   const double u2 = (b >> 11) * (1.0 / 9007199254740992.0);
-  const T v = (T)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
-  out[i] = bf16_exact ? (T)bf16_bits_to_f32(f32_to_bf16_bits((float)v)) : v;
+  const double g = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+  // one rounding from the FP64 draw to BF16 (not FP64 -> FP32 -> BF16)
+  out[i] = bf16_exact ? (T)bf16_bits_to_f32(__builtin_bit_cast(uint16_t, static_cast<__bf16>(g))) : (T)g;
 }
 
 // Column sums of squares of an n x m row-major matrix -> out[m] (FP64).  Sum
