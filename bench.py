@@ -123,6 +123,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-oracle", action="store_true")
+    ap.add_argument("--build-only", type=int, default=0,
+                    help="profiling mode: build the mixed CCCP preconditioner this many times, nothing else")
     ap.add_argument("--matvec-only", type=int, default=0,
                     help="only launch the k=1 matvec this many times per precision (profiling)")
     ap.add_argument("--serving-requests", type=int, default=200,
@@ -164,6 +166,26 @@ def main():
     y_d = y.to(dev).contiguous()
     nr = max(200, int(2 * n ** 0.5))
     stream = torch.cuda.current_stream()
+
+    if args.build_only:
+        # profiling mode: mixed CCCP builds (probe product, Cholesky-QR, CCCP
+        # loop, Newton-Schulz) of the workload's operator, each timed by events
+        op = laker.AttentionKernelOperator(e, lambda_reg=args.lambda_reg, precision="mixed")
+        times = []
+        for _ in range(args.build_only):
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            pre = laker.CCCPPreconditioner(gamma=0.1, verbose=False, probe_source="device",
+                                           probe_precision=args.mixed_probes)
+            ev0.record(stream)
+            pre.build(op.matvec, n, seed=0)
+            ev1.record(stream)
+            ev1.synchronize()
+            times.append(ev0.elapsed_time(ev1))
+            its = pre.iterations
+            pre.close()
+        op.close()
+        print(json.dumps({"build_only": args.build_only, "n": n, "build_ms": times, "cccp_iterations": its}))
+        return
 
     if args.matvec_only:
         # profiling mode: k=1 matvecs of both precisions, the preconditioner
