@@ -71,6 +71,16 @@ __device__ inline float transpose_reduce16(const float (&q)[16], int lane) {
 
 
 using SymPart = float;  // partial sums of the symmetric product
+
+// LDS of a workgroup of the symmetric product (bytes): the double-buffered
+// column chunk (operand rows of pitch lb, then x), the per-tile column sums
+// of the waves (double-buffered; `halves` = 2 keeps the two half-waves' sums
+// apart, 1 joins them in registers first) and x of the block's rows.
+__host__ __device__ constexpr int sym1_lds(int lb, int halves) {
+  return 2 * SYM_JT * 32 * (lb + 4) + (2 * SYM_WAVES * halves * SYM_JT * 32 + SYM_RB) * 4;
+}
+// The half-wave sums stay apart whenever that still fits 64 KB.
+__host__ __device__ constexpr int sym1_halves(int lb) { return sym1_lds(lb, 2) <= 65536 ? 2 : 1; }
 // Waves per SIMD the register allocation must allow: 8-wave workgroups
 // occupy two waves per SIMD each, so a second workgroup per CU needs the
 // kernel within 128 VGPRs (four waves per SIMD; measured 0.77 ms at
@@ -99,10 +109,11 @@ mv_sym1_kernel(const void* __restrict__ colpack, const void* __restrict__ rowpac
   constexpr int gb = MODE == LAKER_PREC_FP32 ? 8 * NS : 32 * NS;  // bytes of one packed row
   constexpr int lb = MODE == LAKER_PREC_FP32 ? ((gb / 4) % 2 == 0 ? gb + 4 : gb) : gb + 16;
   constexpr int chunk = JT * 32 * (lb + 4);
+  constexpr int CBH = sym1_halves(lb);
   char* const buf0 = sym_smem;
   char* const buf1 = sym_smem + chunk;
-  float* const colbuf = reinterpret_cast<float*>(sym_smem + 2 * chunk);  // [2][NW][JT*32]
-  float* const xrow = colbuf + 2 * NW * JT * 32;                         // x of the block's rows
+  float* const colbuf = reinterpret_cast<float*>(sym_smem + 2 * chunk);  // [2][NW][CBH][JT*32]
+  float* const xrow = colbuf + 2 * NW * CBH * JT * 32;                   // x of the block's rows
 
   for (int t = threadIdx.x; t < SYM_RB; t += NT) xrow[t] = R0 + t < n ? X[(R0 + t) * ldx] : 0.f;
   typename G::template Operand<NS> B[ITW];
@@ -177,7 +188,7 @@ mv_sym1_kernel(const void* __restrict__ colpack, const void* __restrict__ rowpac
     const bool more = tc + JT < t1;
     if (more) fetch(tc + JT);
     const int nt = (int)((t1 - tc) < JT ? (t1 - tc) : JT);
-    float* cb = colbuf + (odd ? NW * JT * 32 : 0);
+    float* cb = colbuf + (odd ? NW * CBH * JT * 32 : 0);
     if (active) {
       const char* cur = odd ? buf1 : buf0;
       const float* xs = reinterpret_cast<const float*>(cur + JT * 32 * lb);
@@ -219,11 +230,18 @@ mv_sym1_kernel(const void* __restrict__ colpack, const void* __restrict__ rowpac
           }
         }
         const float cs = q0 + q1;
-        // lanes l and l ^ 32 hold the same column (rows 4h + ...)
-        const auto pr = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(unsigned, cs),
-                                                         __builtin_bit_cast(unsigned, cs), false, false);
-        const float tot = __builtin_bit_cast(float, (unsigned)pr[0]) + __builtin_bit_cast(float, (unsigned)pr[1]);
-        if (h == 0) cb[w * (JT * 32) + tt * 32 + li] = tot;
+        // lanes l and l ^ 32 hold the same column (other rows)
+        if constexpr (CBH == 2) {
+          // both half-waves store their sums; the reduction below adds the
+          // pair (h = 0 first) before it adds the wave's sum to the others
+          cb[(w * 2 + h) * (JT * 32) + tt * 32 + li] = cs;
+        } else {
+          const auto pr = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(unsigned, cs),
+                                                           __builtin_bit_cast(unsigned, cs), false, false);
+          const float tot =
+              __builtin_bit_cast(float, (unsigned)pr[0]) + __builtin_bit_cast(float, (unsigned)pr[1]);
+          if (h == 0) cb[w * (JT * 32) + tt * 32 + li] = tot;
+        }
       };
       for (int tt = 0; tt < nt; ++tt) tile(tt);
     }
@@ -236,9 +254,17 @@ mv_sym1_kernel(const void* __restrict__ colpack, const void* __restrict__ rowpac
       const int64_t j = t * 32 + (threadIdx.x & 31);
       if (t >= tdiag && j < n) {
         // only the last row block has idle waves, and it has no tile right of its diagonal block
-        float v = cb[threadIdx.x];
+        constexpr int S = JT * 32;
+        float v;
+        if constexpr (CBH == 2) {
+          v = cb[threadIdx.x] + cb[S + threadIdx.x];
 #pragma unroll
-        for (int ww = 1; ww < NW; ++ww) v += cb[ww * (JT * 32) + threadIdx.x];
+          for (int ww = 1; ww < NW; ++ww) v += (cb[2 * ww * S + threadIdx.x] + cb[(2 * ww + 1) * S + threadIdx.x]);
+        } else {
+          v = cb[threadIdx.x];
+#pragma unroll
+          for (int ww = 1; ww < NW; ++ww) v += cb[ww * S + threadIdx.x];
+        }
         part_c[sym_coff(b, n) + j - SYM_RB * (b + 1)] = v;
       }
     }
@@ -349,8 +375,7 @@ int launch_sym1(Operator& op, int mode, const void* colpack, const void* rowpack
   // the 16-bit modes (BF16 3-way split, FP16 2-way split) share one layout
   const size_t lbytes = mode == LAKER_PREC_FP32 ? V1Layout<LAKER_PREC_FP32>::lbytes(ka)
                                                 : V1Layout<LAKER_PREC_MIXED>::lbytes(ka);
-  const size_t chunk = (size_t)SYM_JT * 32 * (lbytes + 4);
-  const size_t lds = 2 * chunk + (2 * SYM_WAVES * SYM_JT * 32 + SYM_RB) * sizeof(float);
+  const size_t lds = (size_t)sym1_lds((int)lbytes, sym1_halves((int)lbytes));
   if (lds > 65536 || op.sym_nwg <= 0) return 1;
   const int ns = mode == LAKER_PREC_FP32 ? ka / 2 : ka / 16;
   const bool fp32_ok = ns == 2 || ns == 3 || ns == 4 || ns == 5 || ns == 8 || ns == 16;
