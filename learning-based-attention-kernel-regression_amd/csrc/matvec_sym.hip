@@ -4,6 +4,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <tuple>
+#include <vector>
 
 namespace laker {
 
@@ -72,15 +75,22 @@ __device__ inline float transpose_reduce16(const float (&q)[16], int lane) {
 
 using SymPart = float;  // partial sums of the symmetric product
 
-// LDS of a workgroup of the symmetric product (bytes): the double-buffered
-// column chunk (operand rows of pitch lb, then x), the per-tile column sums
-// of the waves (double-buffered; `halves` = 2 keeps the two half-waves' sums
-// apart, 1 joins them in registers first) and x of the block's rows.
-__host__ __device__ constexpr int sym1_lds(int lb, int halves) {
-  return 2 * SYM_JT * 32 * (lb + 4) + (2 * SYM_WAVES * halves * SYM_JT * 32 + SYM_RB) * 4;
+// LDS of a workgroup of the symmetric product (bytes) for chunks of jt column
+// tiles: the double-buffered column chunk (operand rows of pitch lb, then x),
+// the per-tile column sums of the waves (double-buffered; `halves` = 2 keeps
+// the two half-waves' sums apart, 1 joins them in registers first) and x of
+// the block's rows.
+__host__ __device__ constexpr int sym1_lds(int lb, int jt, int halves) {
+  return 2 * jt * 32 * (lb + 4) + (2 * SYM_WAVES * halves * jt * 32 + SYM_RB) * 4;
 }
-// The half-wave sums stay apart whenever that still fits 64 KB.
-__host__ __device__ constexpr int sym1_halves(int lb) { return sym1_lds(lb, 2) <= 65536 ? 2 : 1; }
+// Two workgroups (four waves per SIMD) share a CU's 160 KB of LDS.  Chunks of
+// 8 column tiles with the half-wave sums apart where that fits, else chunks of
+// 4 with the sums apart, else chunks of 4 with the sums joined.
+constexpr int SYM_LDS_WG = 81920;
+__host__ __device__ constexpr int sym1_jt(int lb) { return sym1_lds(lb, 8, 2) <= SYM_LDS_WG ? 8 : 4; }
+__host__ __device__ constexpr int sym1_halves(int lb) {
+  return sym1_lds(lb, sym1_jt(lb), 2) <= SYM_LDS_WG ? 2 : 1;
+}
 // Waves per SIMD the register allocation must allow: 8-wave workgroups
 // occupy two waves per SIMD each, so a second workgroup per CU needs the
 // kernel within 128 VGPRs (four waves per SIMD; measured 0.77 ms at
@@ -97,7 +107,7 @@ mv_sym1_kernel(const void* __restrict__ colpack, const void* __restrict__ rowpac
                const float* __restrict__ X, int64_t ldx, const int2* __restrict__ map, float* __restrict__ part_r,
                float* __restrict__ part_c, float clampv, const int* __restrict__ skip) {
   using G = GramOps<MODE>;
-  constexpr int ITW = SYM_ITW, NW = SYM_WAVES, NT = 64 * SYM_WAVES, JT = SYM_JT;
+  constexpr int ITW = SYM_ITW, NW = SYM_WAVES, NT = 64 * SYM_WAVES;
   if (skip && *skip) return;  // uniform across the grid
   extern __shared__ __align__(16) char sym_smem[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, h = lane >> 5, li = lane & 31;
@@ -108,6 +118,7 @@ mv_sym1_kernel(const void* __restrict__ colpack, const void* __restrict__ rowpac
   const bool active = I0 < n;
   constexpr int gb = MODE == LAKER_PREC_FP32 ? 8 * NS : 32 * NS;  // bytes of one packed row
   constexpr int lb = MODE == LAKER_PREC_FP32 ? ((gb / 4) % 2 == 0 ? gb + 4 : gb) : gb + 16;
+  constexpr int JT = sym1_jt(lb);
   constexpr int chunk = JT * 32 * (lb + 4);
   constexpr int CBH = sym1_halves(lb);
   char* const buf0 = sym_smem;
@@ -366,6 +377,21 @@ __global__ __launch_bounds__(SRED_ROWS * SRED_TPR) void mv_sym_reduce_kernel(
   }
 }
 
+// Lets `kernel` take `bytes` of dynamic LDS above the 64 KB default (once per
+// device and kernel).
+static int allow_lds(const void* kernel, size_t bytes) {
+  static std::mutex mu;
+  static std::vector<std::tuple<int, const void*, size_t>> done;
+  int dev = 0;
+  LAKER_HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  for (const auto& d : done)
+    if (std::get<0>(d) == dev && std::get<1>(d) == kernel && std::get<2>(d) >= bytes) return LAKER_OK;
+  LAKER_HIP_CHECK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  done.emplace_back(dev, kernel, bytes);
+  return LAKER_OK;
+}
+
 // Symmetric k = 1 product of the square operator (see mv_sym1_kernel) plus
 // its reduction into Y.  Returns 1 when the shape has no instance.
 int launch_sym1(Operator& op, int mode, const void* colpack, const void* rowpack, int ka, int64_t n,
@@ -375,8 +401,9 @@ int launch_sym1(Operator& op, int mode, const void* colpack, const void* rowpack
   // the 16-bit modes (BF16 3-way split, FP16 2-way split) share one layout
   const size_t lbytes = mode == LAKER_PREC_FP32 ? V1Layout<LAKER_PREC_FP32>::lbytes(ka)
                                                 : V1Layout<LAKER_PREC_MIXED>::lbytes(ka);
-  const size_t lds = (size_t)sym1_lds((int)lbytes, sym1_halves((int)lbytes));
-  if (lds > 65536 || op.sym_nwg <= 0) return 1;
+  const int lbi = (int)lbytes;
+  const size_t lds = (size_t)sym1_lds(lbi, sym1_jt(lbi), sym1_halves(lbi));
+  if (lds > SYM_LDS_WG || op.sym_nwg <= 0) return 1;
   const int ns = mode == LAKER_PREC_FP32 ? ka / 2 : ka / 16;
   const bool fp32_ok = ns == 2 || ns == 3 || ns == 4 || ns == 5 || ns == 8 || ns == 16;
   const bool bf16_ok = ns == 1 || ns == 2 || ns == 3 || ns == 4 || ns == 6;
@@ -392,6 +419,10 @@ int launch_sym1(Operator& op, int mode, const void* colpack, const void* rowpack
   dim3 grid((unsigned)op.sym_nwg), block(64 * SYM_WAVES);
 #define LAKER_S1(MODE, NS)                                                                            \
   do {                                                                                                \
+    if (lds > 65536)                                                                                  \
+      LAKER_TRY(allow_lds(clamp ? (const void*)mv_sym1_kernel<MODE, NS, true>                        \
+                                : (const void*)mv_sym1_kernel<MODE, NS, false>,                       \
+                          lds));                                                                      \
     if (clamp)                                                                                        \
       hipLaunchKernelGGL((mv_sym1_kernel<MODE, NS, true>), grid, block, lds, st, colpack, rowpack, n, X, \
                          ldx, map, part_r, part_c, clampv, skip);                                     \
