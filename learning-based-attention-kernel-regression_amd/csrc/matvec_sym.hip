@@ -20,9 +20,10 @@ namespace laker {
 // y_I += P x_J and, right of the diagonal block, the column sums
 // y_J += P^T x_I.  The Gram tile is formed as D = B_row A_col^T, so a lane
 // holds one column (li) and 16 rows (jj_of(v, h)) of each 32 x 32 tile:
-// * column sums are in-lane FMAs over the wave's rows, one swap between the
-//   half-waves (same column, other rows), a fixed-order sum over the waves
-//   in LDS and one FP32 partial per (row block, column);
+// * column sums are in-lane FMAs over the wave's rows; both half-waves (same
+//   column, other rows) store their sums in LDS, a fixed-order sum over the
+//   waves (each wave's two halves first) gives one FP32 partial per (row
+//   block, column);
 // * row sums accumulate per (row, lane column) in FP32 registers over the
 //   whole segment (one FMA per entry, 32 independent chains per lane), and
 //   one 32-lane transpose-reduce per row tile at the end of the segment
