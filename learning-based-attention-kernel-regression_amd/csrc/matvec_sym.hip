@@ -379,18 +379,22 @@ __global__ __launch_bounds__(SRED_ROWS * SRED_TPR) void mv_sym_reduce_kernel(
 }
 
 // Lets `kernel` take `bytes` of dynamic LDS above the 64 KB default (once per
-// device and kernel).
-static int allow_lds(const void* kernel, size_t bytes) {
+// device and kernel).  False when the runtime refuses; the caller then takes
+// another product path.
+static bool allow_lds(const void* kernel, size_t bytes) {
   static std::mutex mu;
   static std::vector<std::tuple<int, const void*, size_t>> done;
   int dev = 0;
-  LAKER_HIP_CHECK(hipGetDevice(&dev));
+  if (hipGetDevice(&dev) != hipSuccess) return false;
   std::lock_guard<std::mutex> lock(mu);
   for (const auto& d : done)
-    if (std::get<0>(d) == dev && std::get<1>(d) == kernel && std::get<2>(d) >= bytes) return LAKER_OK;
-  LAKER_HIP_CHECK(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    if (std::get<0>(d) == dev && std::get<1>(d) == kernel && std::get<2>(d) >= bytes) return true;
+  if (hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
   done.emplace_back(dev, kernel, bytes);
-  return LAKER_OK;
+  return true;
 }
 
 // Symmetric k = 1 product of the square operator (see mv_sym1_kernel) plus
@@ -420,10 +424,10 @@ int launch_sym1(Operator& op, int mode, const void* colpack, const void* rowpack
   dim3 grid((unsigned)op.sym_nwg), block(64 * SYM_WAVES);
 #define LAKER_S1(MODE, NS)                                                                            \
   do {                                                                                                \
-    if (lds > 65536)                                                                                  \
-      LAKER_TRY(allow_lds(clamp ? (const void*)mv_sym1_kernel<MODE, NS, true>                        \
-                                : (const void*)mv_sym1_kernel<MODE, NS, false>,                       \
-                          lds));                                                                      \
+    if (lds > 65536 && !allow_lds(clamp ? (const void*)mv_sym1_kernel<MODE, NS, true>                \
+                                        : (const void*)mv_sym1_kernel<MODE, NS, false>,               \
+                                  lds))                                                               \
+      return 1; /* no room granted: the caller takes the split-K product */                           \
     if (clamp)                                                                                        \
       hipLaunchKernelGGL((mv_sym1_kernel<MODE, NS, true>), grid, block, lds, st, colpack, rowpack, n, X, \
                          ldx, map, part_r, part_c, clampv, skip);                                     \
