@@ -232,6 +232,17 @@ constexpr int PNB = 64;
 // LDS together and the barrier is the only wait.  Sets *info = global index
 // + 1 of the first non-positive pivot (replaced by 1e300 so the factorisation
 // runs on without NaNs).
+//
+// The inverse runs one step behind the factorisation inside the same barrier
+// intervals: column t = i of L^{-1} by forward substitution, column-oriented
+// (once x_p is known, every later row takes v_q -= L[q][p] x_p, so each v_q
+// receives its terms in the order p = 0, 1, ... of the row-oriented dot
+// product).  Step p needs column p of L, published one interval earlier, and
+// its pivot division runs on another wave than the factorisation's square
+// root, so the two chains overlap and the block takes 65 barriers, not 128.
+// v_p stays exactly zero for p < t (zero right-hand side, zero x so far), so
+// x_p = v_p / L[p][p] is zero there without a test.  Wave w holds the rows
+// 4 c + w of the column.
 __device__ __forceinline__ double lane_bcast(double v, int src) {
   const int2 w = __builtin_bit_cast(int2, v);
   return __builtin_bit_cast(double, int2{__builtin_amdgcn_readlane(w.x, src), __builtin_amdgcn_readlane(w.y, src)});
@@ -246,24 +257,36 @@ struct PotrfShared {
   int bad_at;                  // first non-positive pivot (local index) or -1
 };
 
-// Step K of the factorisation for wave w (columns 4 c + w).
+// Barrier interval K for wave w: column K of the factor (K < PNB) and entry
+// P = K - 1 of the inverse columns (K >= 1), then the updates both publish.
 template <int K>
-__device__ __forceinline__ void potrf_fact_step(double (&a)[PW], int i, int w, int nb, PotrfShared& sh) {
-  if ((K & 3) == w) {  // uniform over the wave
-    constexpr int kl = K >> 2;
-    double piv = lane_bcast(a[kl], K);
-    const bool bad = !(piv > 0.0);
-    if (bad && K < nb && i == 0 && sh.bad_at < 0) sh.bad_at = K;
-    piv = bad ? 1e300 : piv;
-    const double d = sqrt(piv);
-    const double r = 1.0 / d;
-    a[kl] = i == K ? d : (i > K ? a[kl] * r : 0.0);
-    sh.col[K & 1][i] = a[kl];
-    sh.Lc[K * (PNB + 1) + i] = a[kl];
+__device__ __forceinline__ void potrf_step(double (&a)[PW], double (&v)[PW], int i, int w, int nb,
+                                           double* __restrict__ dinv, PotrfShared& sh) {
+  constexpr int P = K - 1;
+  if constexpr (K < PNB) {
+    if ((K & 3) == w) {  // uniform over the wave
+      constexpr int kl = K >> 2;
+      double piv = lane_bcast(a[kl], K);
+      const bool bad = !(piv > 0.0);
+      if (bad && K < nb && i == 0 && sh.bad_at < 0) sh.bad_at = K;
+      piv = bad ? 1e300 : piv;
+      const double d = sqrt(piv);
+      const double r = 1.0 / d;
+      a[kl] = i == K ? d : (i > K ? a[kl] * r : 0.0);
+      sh.col[K & 1][i] = a[kl];
+      sh.Lc[K * (PNB + 1) + i] = a[kl];
+    }
+  }
+  if constexpr (P >= 0) {
+    if ((P & 3) == w) {
+      const double xp = v[P >> 2] / sh.Lc[P * (PNB + 1) + P];
+      dinv[P * PNB + i] = xp;
+      sh.xb[P & 1][i] = xp;
+    }
   }
   __syncthreads();
-  constexpr int c1 = K >> 2;  // columns 4 c + w > K: all c > c1, and c = c1 when w > K & 3
-  if constexpr (c1 < PW) {
+  if constexpr (K < PNB) {
+    constexpr int c1 = K >> 2;  // columns 4 c + w > K: all c > c1, and c = c1 when w > K & 3
     const double lik = sh.col[K & 1][i];
     double lj[PW];
 #pragma unroll
@@ -272,21 +295,9 @@ __device__ __forceinline__ void potrf_fact_step(double (&a)[PW], int i, int w, i
 #pragma unroll
     for (int c = c1 + 1; c < PW; ++c) a[c] -= lik * lj[c];
   }
-}
-
-// Step P of the inverse for wave w (rows 4 c + w of column i).
-template <int P>
-__device__ __forceinline__ void potrf_inv_step(double (&v)[PW], int i, int w, double* __restrict__ dinv,
-                                               PotrfShared& sh) {
-  const double* lp = sh.Lc + P * (PNB + 1);
-  if ((P & 3) == w) {
-    const double xp = v[P >> 2] / lp[P];
-    dinv[P * PNB + i] = xp;
-    sh.xb[P & 1][i] = xp;
-  }
-  __syncthreads();
-  constexpr int c1 = P >> 2;
-  if constexpr (c1 < PW) {
+  if constexpr (P >= 0 && (P >> 2) < PW) {
+    constexpr int c1 = P >> 2;
+    const double* lp = sh.Lc + P * (PNB + 1);
     const double xp = sh.xb[P & 1][i];
     double lq[PW];
 #pragma unroll
@@ -299,20 +310,16 @@ __device__ __forceinline__ void potrf_inv_step(double (&v)[PW], int i, int w, do
 
 template <int K>
 struct PotrfSteps {
-  static __device__ __forceinline__ void fact(double (&a)[PW], int i, int w, int nb, PotrfShared& sh) {
-    potrf_fact_step<K>(a, i, w, nb, sh);
-    PotrfSteps<K + 1>::fact(a, i, w, nb, sh);
-  }
-  static __device__ __forceinline__ void inv(double (&v)[PW], int i, int w, double* __restrict__ dinv,
-                                             PotrfShared& sh) {
-    potrf_inv_step<K>(v, i, w, dinv, sh);
-    PotrfSteps<K + 1>::inv(v, i, w, dinv, sh);
+  static __device__ __forceinline__ void run(double (&a)[PW], double (&v)[PW], int i, int w, int nb,
+                                             double* __restrict__ dinv, PotrfShared& sh) {
+    potrf_step<K>(a, v, i, w, nb, dinv, sh);
+    PotrfSteps<K + 1>::run(a, v, i, w, nb, dinv, sh);
   }
 };
 template <>
-struct PotrfSteps<PNB> {
-  static __device__ __forceinline__ void fact(double (&)[PW], int, int, int, PotrfShared&) {}
-  static __device__ __forceinline__ void inv(double (&)[PW], int, int, double* __restrict__, PotrfShared&) {}
+struct PotrfSteps<PNB + 1> {
+  static __device__ __forceinline__ void run(double (&)[PW], double (&)[PW], int, int, int, double* __restrict__,
+                                             PotrfShared&) {}
 };
 
 __global__ __launch_bounds__(256) void potrf_diag_kernel(double* __restrict__ A, int64_t lda, int64_t k0, int nb,
@@ -320,7 +327,7 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double* __restrict__ A,
   __shared__ PotrfShared sh;
   const int i = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (threadIdx.x == 0) sh.bad_at = -1;
-  double a[PW];
+  double a[PW], v[PW];
   {
     const bool rin = i < nb;
     const double* row = A + (k0 + (rin ? i : 0)) * lda + k0;
@@ -329,10 +336,11 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double* __restrict__ A,
       const int cc = 4 * c + w;
       const double g = row[cc < nb ? cc : 0];
       a[c] = (rin && cc < nb) ? g : (i == cc ? 1.0 : 0.0);
+      v[c] = cc == i ? 1.0 : 0.0;
     }
   }
   __syncthreads();  // bad_at initialised
-  PotrfSteps<0>::fact(a, i, w, nb, sh);
+  PotrfSteps<0>::run(a, v, i, w, nb, dinv, sh);
   if (i < nb) {
     double* row = A + (k0 + i) * lda + k0;
 #pragma unroll
@@ -341,16 +349,6 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double* __restrict__ A,
       if (cc < nb) row[cc] = cc <= i ? a[c] : 0.0;
     }
   }
-  // column t = i of L^{-1} by forward substitution, column-oriented: once x_p
-  // is known, every later row takes v_q -= L[q][p] x_p, so each v_q receives
-  // its terms in the order p = 0, 1, ... of the row-oriented dot product.
-  // v_p stays exactly zero for p < t (zero right-hand side, zero x so far),
-  // so x_p = v_p / L[p][p] is zero there without a test.  Wave w holds the
-  // rows 4 c + w of the column.
-  double v[PW];
-#pragma unroll
-  for (int c = 0; c < PW; ++c) v[c] = 4 * c + w == i ? 1.0 : 0.0;
-  PotrfSteps<0>::inv(v, i, w, dinv, sh);
   __syncthreads();
   if (threadIdx.x == 0 && sh.bad_at >= 0 && *info == 0) *info = (int)(k0 + sh.bad_at + 1);
 }
