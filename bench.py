@@ -188,9 +188,8 @@ def main():
         return
 
     if args.matvec_only:
-        # profiling mode: k=1 matvecs of both precisions, the preconditioner
-        # apply, and a plain 256 MiB device copy (the reference point for
-        # scaling hardware byte counters)
+        # profiling mode: k=1 matvecs of both precisions and a plain 256 MiB
+        # device copy (the reference point for scaling hardware byte counters)
         x1 = torch.randn(n, device=dev)
         for prec in ("mixed", "fp32"):
             op = laker.AttentionKernelOperator(e, lambda_reg=args.lambda_reg, precision=prec)
